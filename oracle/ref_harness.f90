@@ -1,0 +1,134 @@
+! ref_harness.f90 -- drives single routines of the reference Fortran sources
+! (compiled unmodified from the reference checkout into _ref/) on a state
+! read from a binary file, and writes the module state back.  Test
+! infrastructure only: used in the build container to pin the C oracle.
+!
+! usage: ref_harness <input.bin> <output.bin>
+! The input layout is written by tests/golden/make_golden.py; kernel ids:
+!   1 primitive  2 primitiveplusbc  3 smooth  4 viscousgp
+!   5 venkatakrishnan  6 source  7 hllcbatten  8 hllczien
+! (7 and 8 with myorder=1 only: the second-order branch calls derivative,
+! which ends in the MPI halo exchange that is not part of this build.)
+program ref_harness
+  use GlobalVariables
+  implicit none
+  external primitive, primitiveplusbc, smooth, viscousgp, venkatakrishnan, source
+  external hllcbatten, hllczien
+  character(len=512) :: fin, fout
+  integer :: hdr(8), kernel, i, u
+  real(dp) :: prm(44)
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(u) hdr
+  nodes = hdr(1); neles = hdr(2); nghosts = hdr(3); myflux = hdr(4)
+  myorder = hdr(5); total = hdr(6); kernel = hdr(7)
+  pghosts = 0; max_pghosts = 0; nprocs = 1; myid = 0
+  ntot = neles + nghosts
+  nepg = neles + nghosts + 1
+  neq = 10; nvar = 33
+  read(u) prm
+  pmax = prm(1); pmin = prm(2); tgmax = prm(3); tgmin = prm(4)
+  ugmax = prm(5); ugmin = prm(6); vgmax = prm(7); vgmin = prm(8); wgmax = prm(9); wgmin = prm(10)
+  upmax = prm(11); upmin = prm(12); vpmax = prm(13); vpmin = prm(14); wpmax = prm(15); wpmin = prm(16)
+  tpmax = prm(17); tpmin = prm(18); pinf = prm(19); gamap = prm(20); gamag = prm(21); cpp = prm(22)
+  cpg = prm(23); amolg = prm(24); prg = prm(25); epslnmax = prm(26); epslnmin = prm(27)
+  epslninf = prm(28); phipzero = prm(29); cfl = prm(30); diap = prm(31); surfacetension = prm(32)
+  as = prm(33); ts = prm(34); co2 = prm(35); co4 = prm(36); c23 = prm(37); cmug = prm(38)
+  rrg = prm(39); gravity = prm(40); prp = prm(41); foursigmabyd = prm(42); dt = prm(43); time = prm(44)
+
+  ! allocations as main.F90:121-229
+  allocate(uq(1), vq(1), wq(1), phiq(1), ur(1), vr(1), wr(1), phir(1), us(1), vs(1), ws(1), phis(1))
+  allocate(vistkg(1), visteg(1), tkg(1), teg(1), delTkgx(1), delTkgy(1), delTkgz(1))
+  allocate(delTegx(1), delTegy(1), delTegz(1), enp(1), yfg(1), yog(1), ypg(1))
+  allocate(visyfg(1), visyog(1), visypg(1))
+  allocate(anu(ntot, 3), x(nodes), y(nodes), z(nodes), xcel(ntot), ycel(ntot), zcel(ntot))
+  allocate(nod(ntot, 4), nc1(ntot), nc2(ntot), nc3(ntot), nc4(ntot), vol(ntot), dl(neles))
+  allocate(sc1x(neles), sc1y(neles), sc1z(neles), sc2x(neles), sc2y(neles), sc2z(neles))
+  allocate(sc3x(neles), sc3y(neles), sc3z(neles), sc4x(neles), sc4y(neles), sc4z(neles))
+  allocate(ug(ntot), vg(ntot), wg(ntot), pg(ntot), pp(ntot), tg(ntot), phig(ntot))
+  allocate(up(ntot), vp(ntot), wp(ntot), phip(ntot), tp(ntot), egtotal(ntot), eptotal(ntot))
+  allocate(rop(ntot), rog(ntot), ag(ntot), ap(ntot))
+  allocate(cn(ntot, 10), rhs(neles, 10), d24(neles, 10), d2q(ntot, 10), cn1(ntot, 10), cu(ntot, 10))
+  allocate(visug(neles), visvg(neles), viswg(neles), vistg(neles), phi(ntot))
+  allocate(visup(neles), visvp(neles), viswp(neles), vistp(neles))
+  allocate(delugx(ntot), delugy(ntot), delugz(ntot), delvgx(ntot), delvgy(ntot), delvgz(ntot))
+  allocate(delwgx(ntot), delwgy(ntot), delwgz(ntot), deltgx(ntot), deltgy(ntot), deltgz(ntot))
+  allocate(delupx(ntot), delupy(ntot), delupz(ntot), delvpx(ntot), delvpy(ntot), delvpz(ntot))
+  allocate(delwpx(ntot), delwpy(ntot), delwpz(ntot), deltpx(ntot), deltpy(ntot), deltpz(ntot))
+  allocate(delpgx(ntot), delpgy(ntot), delpgz(ntot), delppx(ntot), delppy(ntot), delppz(ntot))
+  allocate(delrogx(ntot), delrogy(ntot), delrogz(ntot), delropx(ntot), delropy(ntot), delropz(ntot))
+  allocate(delphigx(ntot), delphigy(ntot), delphigz(ntot))
+  allocate(nparent(nghosts), nghost(nghosts), ntype(nghosts), nside(nghosts))
+  allocate(iblank(ntot), dd(ntot), g_id(ntot))
+
+  read(u) x, y, z
+  read(u) nod
+  read(u) nc1, nc2, nc3, nc4, iblank
+  read(u) nparent, nghost, ntype, nside
+  read(u) xcel, ycel, zcel, vol, dl, dd
+  read(u) sc1x, sc1y, sc1z, sc2x, sc2y, sc2z, sc3x, sc3y, sc3z, sc4x, sc4y, sc4z
+  read(u) cn, cu, cn1, d2q, rhs, d24, anu
+  read(u) ug, vg, wg, pg, pp, tg, phig, up, vp, wp, phip, tp, rop, rog, egtotal, eptotal, ag, ap
+  read(u) delugx, delugy, delugz, delvgx, delvgy, delvgz, delwgx, delwgy, delwgz, &
+          deltgx, deltgy, deltgz, delupx, delupy, delupz, delvpx, delvpy, delvpz, &
+          delwpx, delwpy, delwpz, deltpx, deltpy, deltpz, delpgx, delpgy, delpgz, &
+          delppx, delppy, delppz, delrogx, delrogy, delrogz, delropx, delropy, delropz, &
+          delphigx, delphigy, delphigz
+  read(u) phi
+  read(u) visug, visvg, viswg, vistg, visup, visvp, viswp, vistp
+  close(u)
+
+  ! boundary lists, main.F90:239-275
+  n_bc2 = 0; n_bc3 = 0; n_bc5 = 0; n_bc51 = 0; n_bc53 = 0
+  do i = 1, nghosts
+    select case (ntype(i))
+    case (2);  n_bc2  = n_bc2  + 1
+    case (3);  n_bc3  = n_bc3  + 1
+    case (5);  n_bc5  = n_bc5  + 1
+    case (51); n_bc51 = n_bc51 + 1
+    case (53); n_bc53 = n_bc53 + 1
+    end select
+  end do
+  allocate(bc2_list(n_bc2), bc3_list(n_bc3), bc5_list(n_bc5), bc51_list(n_bc51), bc53_list(n_bc53))
+  n_bc2 = 0; n_bc3 = 0; n_bc5 = 0; n_bc51 = 0; n_bc53 = 0
+  do i = 1, nghosts
+    select case (ntype(i))
+    case (2);  n_bc2 = n_bc2 + 1;   bc2_list(n_bc2) = i
+    case (3);  n_bc3 = n_bc3 + 1;   bc3_list(n_bc3) = i
+    case (5);  n_bc5 = n_bc5 + 1;   bc5_list(n_bc5) = i
+    case (51); n_bc51 = n_bc51 + 1; bc51_list(n_bc51) = i
+    case (53); n_bc53 = n_bc53 + 1; bc53_list(n_bc53) = i
+    end select
+  end do
+
+  select case (kernel)
+  case (1); call primitive
+  case (2); call primitiveplusbc
+  case (3); call smooth
+  case (4); call viscousgp
+  case (5); call venkatakrishnan
+  case (6); call source
+  case (7)
+    if (myorder /= 1) stop 'hllcbatten: harness runs myorder=1 only'
+    call hllcbatten
+  case (8)
+    if (myorder /= 1) stop 'hllczien: harness runs myorder=1 only'
+    call hllczien
+  case default
+    stop 'unknown kernel id'
+  end select
+
+  open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(u) cn, cu, cn1, d2q, rhs, d24, anu
+  write(u) ug, vg, wg, pg, pp, tg, phig, up, vp, wp, phip, tp, rop, rog, egtotal, eptotal, ag, ap
+  write(u) delugx, delugy, delugz, delvgx, delvgy, delvgz, delwgx, delwgy, delwgz, &
+           deltgx, deltgy, deltgz, delupx, delupy, delupz, delvpx, delvpy, delvpz, &
+           delwpx, delwpy, delwpz, deltpx, deltpy, deltpz, delpgx, delpgy, delpgz, &
+           delppx, delppy, delppz, delrogx, delrogy, delrogz, delropx, delropy, delropz, &
+           delphigx, delphigy, delphigz
+  write(u) phi
+  write(u) visug, visvg, viswg, vistg, visup, visvp, viswp, vistp
+  close(u)
+end program ref_harness

@@ -1,0 +1,67 @@
+"""The C oracle against vectors produced by the reference's own Fortran
+routines (tests/golden/make_golden.py, oracle/_ref).  Bit-exact except where
+noted: both sides evaluate the same FP64 expressions in the same order."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from hydraphase import mesh as M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = np.load(os.path.join(HERE, "golden", "ref_kernels.npz"))
+
+ORACLE_CALL = {"primitive": "primitive", "primitiveplusbc": "primitiveplusbc", "smooth": "smooth",
+               "viscousgp": "viscousgp", "venkatakrishnan": "venkat", "source": "source",
+               "hllcbatten": "flux_solver", "hllczien": "flux_solver"}
+
+
+def golden_mesh():
+    f = FIX
+    return M.Mesh(x=f["mesh/x"], y=f["mesh/y"], z=f["mesh/z"], nod=f["mesh/nod"], nc=f["mesh/nc"],
+                  iblank=f["mesh/iblank"], gparent=f["mesh/gparent"], gtype=f["mesh/gtype"],
+                  gside=f["mesh/gside"], shape=tuple(f["mesh/shape"]))
+
+
+def params(flux, myorder):
+    p = M.water_air_params(cfl=0.5, viscous_gas=True, myflux=flux)
+    p["myorder"] = myorder
+    return p
+
+
+def run_case(scen, kernel):
+    flux = 1 if kernel == "hllczien" else 2
+    myorder = 1 if kernel in ("hllcbatten", "hllczien") else 2
+    m = golden_mesh()
+    o = O.Oracle(m, params(flux, myorder))
+    pre = f"{scen}/f{flux}/in/"
+    for n in ["cn", "cu", "cn1", "d2q", "rhs", "d24", "anu", "q", "grad", "phi", "vis"]:
+        o.array(n)[...] = FIX[pre + n]
+    o.set_dt(float(FIX[pre + "dt"]), float(FIX[pre + "time"]), int(FIX[pre + "total"]))
+    o.call(ORACLE_CALL[kernel])
+    return m, o
+
+
+CASES = [(s, k) for s in ("s1", "s3") for k in ORACLE_CALL]
+
+
+@pytest.mark.parametrize("scen,kernel", CASES)
+def test_oracle_matches_reference_routine(scen, kernel):
+    m, o = run_case(scen, kernel)
+    keys = [k for k in FIX.files if k.startswith(f"{scen}/{kernel}/out/")]
+    assert keys
+    ne = m.neles
+    for key in keys:
+        name = key.rsplit("/", 1)[1]
+        ref = FIX[key]
+        got = o.array(name)
+        if name == "q":
+            # ag/ap are only written by primitiveplusbc and the dt loop, and
+            # egtotal/eptotal are dead for the boundary ghosts outside
+            # primitiveplusbc; compare every slot the routine defines
+            ref, got = ref[:16], got[:16]
+        if name in ("cn", "q", "d2q", "anu", "phi"):
+            # interior cells and ghost cells the routine writes
+            pass
+        np.testing.assert_array_equal(got, ref, err_msg=f"{scen}/{kernel}: {name}")
