@@ -1,0 +1,137 @@
+"""Throughput of the HydraPhase per-step FV update on one MI355X.
+
+Workload (the single large case): a 96 x 96 x 90 hex box (unit cross
+section, x in [-0.5,0.5], y in [-0.5,0.5], z in [-0.5,0.5]) Kuhn-split into
+4,976,640 tets with interior nodes jittered by +-10% of the spacing (seed 1),
+the tets inside a z-cylinder of radius 0.1 removed and their faces turned
+into wall ghosts (type 5), gas viscosity on (air 1.8e-5 Pa s via
+Sutherland, water 1e-3 Pa s), a train of 8 water spheres (radii 0.03-0.08,
+seeded) in an air stream at Mach 0.3, CFL 0.5.  FP64 throughout.
+
+A step is one full time step of the reference (main.F90:515-625): primitive,
+CFL reduction, JST smoothing, viscous fluxes and four RK stages of
+primitiveplusbc + gradients + limiter + HLLC face flux + source + update.
+The metric counts cell-stage updates (active cells x 4 stages) per second
+with the state resident in HBM.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "multiphase-code_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes of one face-flux launch per active cell (FP64): own
+# primitives 14x8, gradients 33x8, limiter 8, area vectors 12x8, face
+# centroids 12x8, centroid 3x8, neighbour ids 4x4, vol+dl 16, viscous flux
+# 8x8, d24 10x8, cu 10x8, cn read (stages 2-4) 3/4x10x8, cn1 read (stage 4)
+# and write (stage 1) 2x1/4x10x8, cn write 10x8.  Neighbour values are
+# counted once, by their owning cell.
+FLUX_BYTES_PER_CELL = 112 + 264 + 8 + 96 + 96 + 24 + 16 + 16 + 64 + 80 + 80 + 60 + 40 + 80
+
+
+def build_case(nx, ny, nz, seed=1):
+    from hydraphase import mesh as M
+    hole = (0.0, 0.0, 0.1)
+    m = M.box_mesh(nx, ny, nz, jitter=0.1, seed=seed, hole=hole)
+    p = M.water_air_params(cfl=0.5, viscous_gas=True)
+    spheres = M.sphere_train(seed, n=8, rmin=0.03, rmax=0.08, avoid=hole)
+    cu, gp = M.sphere_state(m, p, spheres)
+    return m, p, cu, gp
+
+
+def cpu_baseline(m, p, cu, gp, seconds_target=20.0):
+    """The serial C oracle (a port of the reference's FP64 path, 1 thread)
+    on a bounded sample of the same workload: whole steps of the same case
+    until about seconds_target of CPU time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    o = O.Oracle(m, p)
+    o.set_state(cu, gp, 0.0, 0)
+    nsteps, t0 = 0, time.perf_counter()
+    while True:
+        o.step()
+        nsteps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds_target or nsteps >= 50:
+            break
+    active = int((m.iblank == 1).sum())
+    return {"value": active * 4 * nsteps / el, "unit": "cell-stage updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{nsteps} full step(s) of this workload ({active} cells) on the C oracle, "
+                      f"{el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, nargs=3, default=[96, 96, 90])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every case runs "
+              "on one device; the multi-GPU decomposition is a design only)", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.time()
+    m, p, cu, gp = build_case(*args.n)
+    print(f"# case built: {m.neles} tets, {m.nghosts} ghosts ({time.time() - t0:.1f} s)", file=sys.stderr,
+          flush=True)
+    from hydraphase import Solver
+    s = Solver(m, p)
+    s.upload(cu, gp, 0.0, 0)
+    if args.warmup:
+        s.step(args.warmup)
+        s.synchronize()
+    s.step(args.steps)
+    s.synchronize()
+    total_ms, flux_ms, nflux = s.last_timing()
+    cu_out, _, tphys, tot = s.download()
+    if not np.isfinite(cu_out).all():
+        print("non-finite state after the timed steps", file=sys.stderr)
+        sys.exit(1)
+    active = int((m.iblank == 1).sum())
+    updates = active * 4 * args.steps
+    ms_per_step = total_ms / args.steps
+    value = updates / (total_ms * 1e-3)
+    flux_avg_ms = flux_ms / max(nflux, 1)
+    achieved = FLUX_BYTES_PER_CELL * active / (flux_avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "cell-stage updates/s @5M tets",
+        "value": value,
+        "unit": "cell-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"box {args.n[0]}x{args.n[1]}x{args.n[2]} hexes x6 tets, cylinder wall "
+                               f"r=0.1, 8 water spheres, viscous, FP64",
+                   "tets": m.neles, "active_cells": active, "ghosts": m.nghosts, "stages_per_step": 4},
+        "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "flux_avg_ms": flux_avg_ms, "bytes_per_cell": FLUX_BYTES_PER_CELL},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(m, p, cu, gp, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

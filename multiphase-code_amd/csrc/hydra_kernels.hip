@@ -1,0 +1,1197 @@
+// hydra_kernels.hip -- FP64 HIP kernels of the HydraPhase per-step finite
+// volume update for MI355X (gfx950, wave64).
+//
+// One thread owns one tetrahedral cell and evaluates that cell's own four
+// faces, as the reference does (sr_hllcbatten.F90:59-556): the HLLC flux
+// carries non-antisymmetric regularisers (+1.0E-6 in the star-state
+// denominators, +1.0E-8 in the contact speed), so a face-once scatter would
+// not reproduce the reference's residuals.  Residuals are gathered in
+// registers, never scattered: no atomics, bitwise-deterministic results.
+//
+// Expressions follow the reference's left-to-right FP64 evaluation, with its
+// REAL*4 literals promoted to double; the library is built with
+// -ffp-contract=off so no multiply-add is fused.  Citations are relative to
+// HydraPhase_60_FPC_5mill/src/.
+//
+// Data layout in HBM (structure of arrays, variable-major, leading dimension
+// neles unless noted): conserved cn/cu/cn1 (10), primitives q (14 x ntot:
+// interior cells then boundary ghosts), gradients (33), limiter phi, viscous
+// fluxes (8), JST d24 (10).  Boundary ghosts are reached through nc >= neles;
+// the JST routine reads a ghost through its parent (nc_par) because the
+// reference copies parent values into the ghost right before using them
+// (sr_smooth.F90:19-41, :164-187).
+
+#include <hip/hip_runtime.h>
+
+#include "hydra_dev.h"
+
+#define F1E8 ((double)1.0e-8f)
+#define F1E6 ((double)1.0e-6f)
+#define F0P1 ((double)0.1f)
+
+namespace hydra {
+
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+
+// mod_MatrixOps.F90:4-9
+__device__ __forceinline__ double det3(double a11, double a21, double a31, double a12, double a22,
+                                       double a32, double a13, double a23, double a33)
+{
+    return a11 * (a22 * a33 - a23 * a32) - a21 * (a12 * a33 - a13 * a32) + a31 * (a12 * a23 - a13 * a22);
+}
+
+// mod_MatrixOps.F90:12-18
+__device__ __forceinline__ double fvenkat(double delp, double delm, double epssq)
+{
+    return (((delp * delp + epssq) * delm + 2.0 * delm * delm * delp) /
+            (delp * delp + 2.0 * delm * delm + delp * delm + epssq)) / delm;
+}
+
+#define QV(s, i) q[(size_t)(s) * nt + (i)]
+
+// ---------------------------------------------------------------------------
+// geometry (sr_geometry.F90:147-289), once per mesh
+// ---------------------------------------------------------------------------
+__global__ void k_geom_cell(Mesh m)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.ne) return;
+    const int ne = m.ne;
+    const double *x = m.x, *y = m.y, *z = m.z;
+    int n[4];
+    for (int k = 0; k < 4; k++) n[k] = m.nod[(size_t)k * ne + i];
+    // area vectors, face node triples of sr_geometry.F90:149-158
+    const int tri[4][3] = {{1, 2, 3}, {3, 2, 0}, {1, 3, 0}, {2, 1, 0}};
+    for (int f = 0; f < 4; f++) {
+        const int n1 = n[tri[f][0]], n2 = n[tri[f][1]], n3 = n[tri[f][2]];
+        double dxyA = (x[n1] - x[n2]) * (y[n1] + y[n2]);
+        double dxyB = (x[n2] - x[n3]) * (y[n2] + y[n3]);
+        double dxyC = (x[n3] - x[n1]) * (y[n3] + y[n1]);
+        double dyzA = (y[n1] - y[n2]) * (z[n1] + z[n2]);
+        double dyzB = (y[n2] - y[n3]) * (z[n2] + z[n3]);
+        double dyzC = (y[n3] - y[n1]) * (z[n3] + z[n1]);
+        double dzxA = (z[n1] - z[n2]) * (x[n1] + x[n2]);
+        double dzxB = (z[n2] - z[n3]) * (x[n2] + x[n3]);
+        double dzxC = (z[n3] - z[n1]) * (x[n3] + x[n1]);
+        m.sc[(size_t)(3 * f + 0) * ne + i] = 0.5 * (dyzA + dyzB + dyzC);
+        m.sc[(size_t)(3 * f + 1) * ne + i] = 0.5 * (dzxA + dzxB + dzxC);
+        m.sc[(size_t)(3 * f + 2) * ne + i] = 0.5 * (dxyA + dxyB + dxyC);
+    }
+    const double x1 = x[n[0]], x2 = x[n[1]], x3 = x[n[2]], x4 = x[n[3]];
+    const double y1 = y[n[0]], y2 = y[n[1]], y3 = y[n[2]], y4 = y[n[3]];
+    const double z1 = z[n[0]], z2 = z[n[1]], z3 = z[n[2]], z4 = z[n[3]];
+    m.xc[i] = 0.25 * (x1 + x2 + x3 + x4);
+    m.yc[i] = 0.25 * (y1 + y2 + y3 + y4);
+    m.zc[i] = 0.25 * (z1 + z2 + z3 + z4);
+    double d12 = sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2) + (z1 - z2) * (z1 - z2));
+    double d13 = sqrt((x1 - x3) * (x1 - x3) + (y1 - y3) * (y1 - y3) + (z1 - z3) * (z1 - z3));
+    double d14 = sqrt((x1 - x4) * (x1 - x4) + (y1 - y4) * (y1 - y4) + (z1 - z4) * (z1 - z4));
+    double d23 = sqrt((x2 - x3) * (x2 - x3) + (y2 - y3) * (y2 - y3) + (z2 - z3) * (z2 - z3));
+    double d42 = sqrt((x4 - x2) * (x4 - x2) + (y4 - y2) * (y4 - y2) + (z4 - z2) * (z4 - z2));
+    double d43 = sqrt((x4 - x3) * (x4 - x3) + (y4 - y3) * (y4 - y3) + (z4 - z3) * (z4 - z3));
+    double dl = dmin(dmin(dmin(dmin(dmin(d12, d13), d14), d23), d42), d43);
+    m.dl[i] = dl;
+    double v6 = det3(x2, x3, x4, y2, y3, y4, z2, z3, z4) - det3(x1, x3, x4, y1, y3, y4, z1, z3, z4) +
+                det3(x1, x2, x4, y1, y2, y4, z1, z2, z4) - det3(x1, x2, x3, y1, y2, y3, z1, z2, z3);
+    m.vol[i] = fabs(v6) / 6.0;
+    // limiter threshold (0.3*dl)**3.0d0, evaluated as (t*t)*t (sr_VK.F90:34)
+    const double t = 0.3 * dl;
+    m.epssq[i] = t * t * t;
+    // face centroids: flux/limiter variant (sum order of sr_hllcbatten.F90:79-119,
+    // times 0.3333D0) and viscous variant (sr_viscousgp.F90:499-530, / 3.0d0)
+    const int fl[4][3] = {{1, 3, 2}, {2, 3, 0}, {3, 1, 0}, {1, 2, 0}};
+    const int fv[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    for (int f = 0; f < 4; f++) {
+        const int a = n[fl[f][0]], b = n[fl[f][1]], d = n[fl[f][2]];
+        m.fc0[(size_t)(3 * f + 0) * ne + i] = (x[a] + x[b] + x[d]) * 0.3333;
+        m.fc0[(size_t)(3 * f + 1) * ne + i] = (y[a] + y[b] + y[d]) * 0.3333;
+        m.fc0[(size_t)(3 * f + 2) * ne + i] = (z[a] + z[b] + z[d]) * 0.3333;
+        const int a3 = n[fv[f][0]], b3 = n[fv[f][1]], d3 = n[fv[f][2]];
+        m.fc3[(size_t)(3 * f + 0) * ne + i] = (x[a3] + x[b3] + x[d3]) / 3.0;
+        m.fc3[(size_t)(3 * f + 1) * ne + i] = (y[a3] + y[b3] + y[d3]) / 3.0;
+        m.fc3[(size_t)(3 * f + 2) * ne + i] = (z[a3] + z[b3] + z[d3]) / 3.0;
+    }
+}
+
+// ghost centroid mirrored through the boundary face; ghost volume = parent's
+// (sr_geometry.F90:245-289).  dd, the distance used by the relaxing BCs, is
+// written by the ghost the host marked as the last one of its parent with
+// type 3/51/53 in list order (sr_geometry.F90:295-347 overwrites).
+__global__ void k_geom_ghost(Mesh m, double xmax, double ymax, double zmax, double ymin, double zmin)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= m.ng) return;
+    const int ne = m.ne, np = m.gpar[g], ng = ne + g, ns = m.gside[g];
+    const double *x = m.x, *y = m.y, *z = m.z;
+    int n[4];
+    for (int k = 0; k < 4; k++) n[k] = m.nod[(size_t)k * ne + np];
+    const int tri[4][3] = {{1, 2, 3}, {3, 2, 0}, {1, 3, 0}, {2, 1, 0}};
+    const int n1 = n[tri[ns - 1][0]], n2 = n[tri[ns - 1][1]], n3 = n[tri[ns - 1][2]];
+    double xcf = (x[n1] + x[n2] + x[n3]) / 3.0;
+    double ycf = (y[n1] + y[n2] + y[n3]) / 3.0;
+    double zcf = (z[n1] + z[n2] + z[n3]) / 3.0;
+    m.xc[ng] = 2.0 * xcf - m.xc[np];
+    m.yc[ng] = 2.0 * ycf - m.yc[np];
+    m.zc[ng] = 2.0 * zcf - m.zc[np];
+    m.vol[ng] = m.vol[np];
+    if (m.gdd_win[g]) {
+        const int nt = m.gtype[g];
+        if (nt == 3) {
+            double c = (x[n[0]] + x[n[1]] + x[n[2]] + x[n[3]]) * 0.25;
+            m.dd[np] = fabs(xmax - c);
+        } else if (nt == 51) {
+            double c = (y[n[0]] + y[n[1]] + y[n[2]] + y[n[3]]) * 0.25;
+            m.dd[np] = c > 0.0 ? fabs(ymax - c) : fabs(ymin - c);
+        } else if (nt == 53) {
+            double c = (z[n[0]] + z[n[1]] + z[n[2]] + z[n[3]]) * 0.25;
+            m.dd[np] = c > F0P1 ? fabs(zmax - c) : fabs(zmin - c);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// primitive recovery: mixture-pressure quadratic, phase clipping and cubic
+// blending, conserved rewrite (sr_primitive.F90:23-161,
+// sr_primitiveplusbc.F90:36-172).  Optionally the CFL bound of
+// main.F90:541-545 reduced to one minimum per block with wave shuffles.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void eos_cell(const Params &P, double *cn, int ne, int i, double *q, int nt,
+                                         double *dtloc, const double *dl)
+{
+    const double c1 = cn[i], c2 = cn[(size_t)ne + i], c3 = cn[(size_t)2 * ne + i], c4 = cn[(size_t)3 * ne + i],
+                 c5 = cn[(size_t)4 * ne + i], c6 = cn[(size_t)5 * ne + i], c7 = cn[(size_t)6 * ne + i],
+                 c8 = cn[(size_t)7 * ne + i], c9 = cn[(size_t)8 * ne + i], c10 = cn[(size_t)9 * ne + i];
+    double ug = c2 / c1, vg = c3 / c1, wg = c4 / c1;
+    double up = c7 / c6, vp = c8 / c6, wp = c9 / c6;
+    double eg = c5 / c1 - 0.5 * (ug * ug + vg * vg + wg * wg);
+    double ep = c10 / c6 - 0.5 * (up * up + vp * vp + wp * wp);
+    const double gm1 = P.gamag - 1.0, pm1 = P.gamap - 1.0;
+    const double sb = P.foursigmabyd + P.pinf * P.gamap;
+    double bbb = sb - (c1 * gm1 * eg) - (c6 * pm1 * ep);
+    double ccc = (c1 * gm1 * eg * sb);
+    double pg = (-bbb + sqrt(bbb * bbb + 4.0 * ccc)) * 0.5;
+    double pp = pg + P.foursigmabyd;
+    double phig = (c1 * gm1 * eg) / pg;
+    double phip = 1.0 - phig;
+    double rog = c1 / phig;
+    double rop = c6 / phip;
+    double tg = pg / (rog * P.rrg);
+    double tp = ((pp + P.pinf) * P.gamap) / (pm1 * rop * P.cpp);
+    if (phig < P.epslnmin) {
+        phig = P.epslnmin; phip = 1.0 - phig;
+        ug = up; vg = vp; wg = wp; tg = tp; pg = pp;
+    }
+    if (phip < P.epslnmin) {
+        phip = P.epslnmin; phig = 1.0 - phip;
+        up = ug; vp = vg; wp = wg; tp = tg; pp = pg;
+    }
+    if (phig >= P.epslnmin && phig <= P.epslnmax) {
+        double psig = (phig - P.epslnmin) / (P.epslnmax - P.epslnmin);
+        double fg = -psig * psig * (2.0 * psig - 3.0);
+        ug = fg * ug + (1.0 - fg) * up;
+        vg = fg * vg + (1.0 - fg) * vp;
+        wg = fg * wg + (1.0 - fg) * wp;
+        tg = fg * tg + (1.0 - fg) * tp;
+        pg = fg * pg + (1.0 - fg) * pp;
+    }
+    if (phip >= P.epslnmin && phip <= P.epslnmax) {
+        double psig = (phip - P.epslnmin) / (P.epslnmax - P.epslnmin);
+        double fg = -psig * psig * (2.0 * psig - 3.0);
+        up = fg * up + (1.0 - fg) * ug;
+        vp = fg * vp + (1.0 - fg) * vg;
+        wp = fg * wp + (1.0 - fg) * wg;
+        tp = fg * tp + (1.0 - fg) * tg;
+        pp = fg * pp + (1.0 - fg) * pg;
+    }
+    rog = pg / (P.rrg * tg);
+    rop = (pp + P.pinf) * P.gamap / (pm1 * P.cpp * tp);
+    eg = pg / (gm1 * rog);
+    ep = (pp + P.gamap * P.pinf) / (pm1 * rop);
+    const double egt = eg + 0.5 * (ug * ug + vg * vg + wg * wg);
+    const double ept = ep + 0.5 * (up * up + vp * vp + wp * wp);
+    QV(Q_UG, i) = ug; QV(Q_VG, i) = vg; QV(Q_WG, i) = wg; QV(Q_PG, i) = pg; QV(Q_PP, i) = pp;
+    QV(Q_TG, i) = tg; QV(Q_PHIG, i) = phig; QV(Q_UP, i) = up; QV(Q_VP, i) = vp; QV(Q_WP, i) = wp;
+    QV(Q_PHIP, i) = phip; QV(Q_TP, i) = tp; QV(Q_ROP, i) = rop; QV(Q_ROG, i) = rog;
+    const double mg = phig * rog, mp = phip * rop;
+    cn[i] = mg;
+    cn[(size_t)ne + i] = mg * ug;
+    cn[(size_t)2 * ne + i] = mg * vg;
+    cn[(size_t)3 * ne + i] = mg * wg;
+    cn[(size_t)4 * ne + i] = mg * egt;
+    cn[(size_t)5 * ne + i] = mp;
+    cn[(size_t)6 * ne + i] = mp * up;
+    cn[(size_t)7 * ne + i] = mp * vp;
+    cn[(size_t)8 * ne + i] = mp * wp;
+    cn[(size_t)9 * ne + i] = mp * ept;
+    if (dtloc) {
+        // main.F90:541-545
+        double ag = sqrt((P.gamag * pg) / rog);
+        double ap = sqrt((P.gamap * (pp + P.pinf)) / rop);
+        double qgi = sqrt(ug * ug + vg * vg + wg * wg);
+        double qpi = sqrt(up * up + vp * vp + wp * wp);
+        *dtloc = dl[i] / (dmax(ag, ap) + dmax(qgi, qpi));
+    }
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_eos(Params P, State s, int passes, double *dtpart)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double loc = 1.0e8;
+    if (i < s.ne) {
+        for (int k = 0; k < passes; k++)
+            eos_cell(P, s.cn, s.ne, i, s.q, s.nt, (dtpart && k == passes - 1) ? &loc : nullptr, s.dl);
+    }
+    if (dtpart) {
+        __shared__ double red[4];
+        double v = wave_min(loc);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = red[0];
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = dmin(r, red[w]);
+            dtpart[blockIdx.x] = r;
+        }
+    }
+}
+
+// dtmin over the block minima; dt = cfl*dtmin, time += dt, total += 1
+// (main.F90:520-523, :555-556).  One block of 256 threads.
+__global__ void __launch_bounds__(256) k_dt_final(Params P, State s, const double *dtpart, int nparts)
+{
+    double v = 1.0e8;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) v = dmin(v, dtpart[b]);
+    __shared__ double red[4];
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int w = 1; w < 4; w++) r = dmin(r, red[w]);
+        double dt = r * P.cfl;
+        s.scal[SC_DTMIN] = r;
+        s.scal[SC_DT] = dt;
+        s.scal[SC_TIME] = s.scal[SC_TIME] + dt;
+        s.itot[0] = s.itot[0] + 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// boundary ghosts (sr_primitiveplusbc.F90:198-592; viscous variant
+// sr_viscousgp.F90:71-446).  Each ghost owns its slot, so the five
+// reference loops collapse into one launch with a switch on the type.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_bc(Params P, State s, Mesh m, int viscous)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= m.ng) return;
+    const int ne = m.ne, nt = s.nt, np = m.gpar[g], ng = ne + g, type = m.gtype[g];
+    double *q = s.q;
+    const double dt = s.scal[SC_DT];
+    const int total = s.itot[0];
+    if (type == 2) {
+        QV(Q_PHIP, ng) = P.phipzero;
+        QV(Q_PHIG, ng) = 1.0 - P.phipzero;
+        QV(Q_UG, ng) = P.ugmin; QV(Q_VG, ng) = P.vgmin; QV(Q_WG, ng) = P.wgmin;
+        const double tg = P.tgmin;
+        QV(Q_TG, ng) = tg;
+        const double pg = QV(Q_PG, np);
+        QV(Q_PG, ng) = pg;
+        QV(Q_ROG, ng) = pg / (P.rrg * tg);
+        QV(Q_UP, ng) = 0.99 * P.ugmin; QV(Q_VP, ng) = 0.99 * P.vgmin; QV(Q_WP, ng) = 0.99 * P.wgmin;
+        QV(Q_TP, ng) = P.tgmin;
+        QV(Q_PP, ng) = QV(Q_PP, np);
+        QV(Q_ROP, ng) = P.gamap * (pg + P.pinf) / ((P.gamap - 1.0) * P.cpp * P.tgmin);
+        return;
+    }
+    if (type == 5) {
+        QV(Q_PHIG, ng) = QV(Q_PHIG, np);
+        QV(Q_UG, ng) = -QV(Q_UG, np); QV(Q_VG, ng) = -QV(Q_VG, np); QV(Q_WG, ng) = -QV(Q_WG, np);
+        const double tg = QV(Q_TG, np), tp = QV(Q_TP, np), pg = QV(Q_PG, np), pp = QV(Q_PP, np);
+        QV(Q_TG, ng) = tg;
+        QV(Q_PHIP, ng) = QV(Q_PHIP, np);
+        QV(Q_UP, ng) = -QV(Q_UP, np); QV(Q_VP, ng) = -QV(Q_VP, np); QV(Q_WP, ng) = -QV(Q_WP, np);
+        QV(Q_TP, ng) = tp;
+        QV(Q_PG, ng) = pg; QV(Q_PP, ng) = pp;
+        QV(Q_ROG, ng) = pg / (P.rrg * tg);
+        QV(Q_ROP, ng) = P.gamap * (pp + P.pinf) / ((P.gamap - 1.0) * P.cpp * tp);
+        return;
+    }
+    const double dd = m.dd[np];
+    if (type == 3) {
+        QV(Q_PHIG, ng) = QV(Q_PHIG, np);
+        QV(Q_PHIP, ng) = QV(Q_PHIP, np);
+        const double ugp = QV(Q_UG, np), vgp = QV(Q_VG, np), wgp = QV(Q_WG, np), tgp = QV(Q_TG, np);
+        const double wT = ugp;
+        const double wU = ugp + sqrt(P.gamag * QV(Q_PG, np) / QV(Q_ROG, np));
+        const double alphaT = wT * dt / dd;
+        const double alphaU = wU * dt / dd;
+        const double fracT = 1.0 / (1.0 + alphaT);
+        const double fracU = 1.0 / (1.0 + alphaU);
+        double ugg = QV(Q_UG, ng), vgg = QV(Q_VG, ng), wgg = QV(Q_WG, ng), tgg = QV(Q_TG, ng);
+        if (total == 1) { ugg = ugp; vgg = vgp; wgg = wgp; tgg = tgp; }
+        if (ugp > ugg) {
+            ugg = ugg * fracU + (1.0 - fracU) * (2.0 * ugp - ugg);
+            vgg = vgg * fracT + (1.0 - fracT) * (2.0 * vgp - vgg);
+            wgg = wgg * fracT + (1.0 - fracT) * (2.0 * wgp - wgg);
+            tgg = tgg * fracT + (1.0 - fracT) * (2.0 * tgp - tgg);
+        } else {
+            ugg = ugp; vgg = vgp; wgg = wgp; tgg = tgp;
+        }
+        QV(Q_UG, ng) = ugg; QV(Q_VG, ng) = vgg; QV(Q_WG, ng) = wgg; QV(Q_TG, ng) = tgg;
+        QV(Q_PG, ng) = P.pmin;
+        QV(Q_ROG, ng) = P.pmin / (P.rrg * tgg);
+        QV(Q_VP, ng) = QV(Q_VP, np); QV(Q_UP, ng) = QV(Q_UP, np); QV(Q_WP, ng) = QV(Q_WP, np);
+        QV(Q_PP, ng) = P.pmin;
+        const double tp = QV(Q_TP, np);
+        QV(Q_TP, ng) = tp;
+        QV(Q_ROP, ng) = P.gamap * (P.pmin + P.pinf) / ((P.gamap - 1.0) * P.cpp * tp);
+        return;
+    }
+    if (type == 51 || type == 53) {
+        // tangential relaxation toward the mirrored state, normal velocity
+        // reflected (:425-591).  51 reflects v, 53 reflects w; in the viscous
+        // copy the 53 branch negates the ghost's own w (sr_viscousgp.F90:408).
+        const bool y = (type == 51);
+        const double ugp = QV(Q_UG, np), vgp = QV(Q_VG, np), wgp = QV(Q_WG, np), tgp = QV(Q_TG, np);
+        const double nrm = y ? vgp : wgp;
+        const double alphaT = nrm * dt / dd;
+        const double fracT = 1.0 / (1.0 + alphaT);
+        double ugg = QV(Q_UG, ng), vgg = QV(Q_VG, ng), wgg = QV(Q_WG, ng), tgg = QV(Q_TG, ng);
+        if (y) {
+            if (vgp > vgg) {
+                vgg = -vgp;
+                ugg = ugg * fracT + (1.0 - fracT) * (2.0 * ugp - ugg);
+                wgg = wgg * fracT + (1.0 - fracT) * (2.0 * wgp - wgg);
+                tgg = tgg * fracT + (1.0 - fracT) * (2.0 * tgp - tgg);
+            } else {
+                ugg = ugp; vgg = -vgp; wgg = wgp; tgg = tgp;
+            }
+        } else {
+            if (wgp > wgg) {
+                wgg = viscous ? -wgg : -wgp;
+                ugg = ugg * fracT + (1.0 - fracT) * (2.0 * ugp - ugg);
+                vgg = vgg * fracT + (1.0 - fracT) * (2.0 * vgp - vgg);
+                tgg = tgg * fracT + (1.0 - fracT) * (2.0 * tgp - tgg);
+            } else {
+                ugg = ugp; vgg = vgp; wgg = -wgp; tgg = tgp;
+            }
+        }
+        QV(Q_UG, ng) = ugg; QV(Q_VG, ng) = vgg; QV(Q_WG, ng) = wgg; QV(Q_TG, ng) = tgg;
+        const double pg = QV(Q_PG, np), pp = QV(Q_PP, np), tp = QV(Q_TP, np);
+        QV(Q_PG, ng) = pg;
+        QV(Q_PHIG, ng) = QV(Q_PHIG, np);
+        QV(Q_PHIP, ng) = QV(Q_PHIP, np);
+        QV(Q_UP, ng) = QV(Q_UP, np);
+        QV(Q_VP, ng) = y ? -QV(Q_VP, np) : QV(Q_VP, np);
+        QV(Q_WP, ng) = y ? QV(Q_WP, np) : -QV(Q_WP, np);
+        QV(Q_TP, ng) = tp;
+        QV(Q_PP, ng) = pp;
+        QV(Q_ROG, ng) = pg / (P.rrg * tgg);
+        QV(Q_ROP, ng) = P.gamap * (pp + P.pinf) / ((P.gamap - 1.0) * P.cpp * tp);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// JST dissipation (sr_smooth.F90): pass 1 switches anu + undivided Laplacian
+// d2q, pass 2 the blended 2nd/4th difference d24.  Ghost values are the
+// parent's (nc_par).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double switch_sum(const double *a, int i, const int *nb, double *den)
+{
+    const double c = a[i];
+    double n = fabs(a[nb[0]] - c) + fabs(a[nb[1]] - c) + fabs(a[nb[2]] - c) + fabs(a[nb[3]] - c);
+    *den = fabs(a[nb[0]] + c) + fabs(a[nb[1]] + c) + fabs(a[nb[2]] + c) + fabs(a[nb[3]] + c);
+    return n;
+}
+
+__global__ void __launch_bounds__(256) k_smooth1(State s, Mesh m)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ne = m.ne;
+    if (i >= ne) return;
+    int nb[4];
+    for (int k = 0; k < 4; k++) nb[k] = m.ncp[(size_t)k * ne + i];
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    double den, num;
+    num = switch_sum(q + (size_t)Q_PG * nt, i, nb, &den);
+    const double anupg = num / den;
+    num = switch_sum(s.cn + (size_t)4 * ne, i, nb, &den);
+    double a1 = anupg + num / (den + F1E6);
+    num = switch_sum(s.cn, i, nb, &den);
+    a1 = a1 + num / (den + F1E6);
+    num = switch_sum(q + (size_t)Q_PP * nt, i, nb, &den);
+    const double anupp = num / den;
+    num = switch_sum(s.cn + (size_t)5 * ne, i, nb, &den);
+    double a2 = anupp + num / (den + F1E6);
+    num = switch_sum(s.cn + (size_t)9 * ne, i, nb, &den);
+    a2 = a2 + num / (den + F1E6);
+    s.anu[i] = a1;
+    s.anu[(size_t)ne + i] = a2;
+    for (int k = 0; k < 10; k++) {
+        const double *c = s.cn + (size_t)k * ne;
+        const double ci = c[i];
+        s.d2q[(size_t)k * ne + i] = c[nb[0]] - ci + c[nb[1]] - ci + c[nb[2]] - ci + c[nb[3]] - ci;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_smooth2(Params P, State s, Mesh m)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ne = m.ne;
+    if (i >= ne) return;
+    int nb[4];
+    for (int k = 0; k < 4; k++) nb[k] = m.ncp[(size_t)k * ne + i];
+    const double dt = s.scal[SC_DT];
+    const double vi = m.vol[i];
+    double d24n[4];
+    for (int f = 0; f < 4; f++) d24n[f] = 0.5 * (vi / dt + m.vol[nb[f]] / dt);
+    for (int set = 0; set < 2; set++) {
+        const double *anu = s.anu + (size_t)set * ne;
+        double d2c[4], d4c[4];
+        for (int f = 0; f < 4; f++) {
+            const double e2 = P.co2 * dmax(anu[i], anu[nb[f]]);
+            const double e4 = dmax(0.0, P.co4 - e2);
+            d2c[f] = e2 * d24n[f];
+            d4c[f] = e4 * d24n[f];
+        }
+        for (int k = 5 * set; k < 5 * set + 5; k++) {
+            const double *c = s.cn + (size_t)k * ne;
+            const double *w = s.d2q + (size_t)k * ne;
+            const double ci = c[i], wi = w[i];
+            double d2 = d2c[0] * (c[nb[0]] - ci) + d2c[1] * (c[nb[1]] - ci) + d2c[2] * (c[nb[2]] - ci) +
+                        d2c[3] * (c[nb[3]] - ci);
+            double d4 = d4c[0] * (w[nb[0]] - wi) + d4c[1] * (w[nb[1]] - wi) + d4c[2] * (w[nb[2]] - wi) +
+                        d4c[3] * (w[nb[3]] - wi);
+            s.d24[(size_t)k * ne + i] = d2 - d4;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// viscous face fluxes (sr_viscousgp.F90:450-1139, AMUP_CONSTANT build):
+// least-squares gradient at each face centroid from the cell's 4 neighbours
+// plus the cell (boundary face) or plus the neighbour's 4 neighbours
+// (interior face); Newtonian stress and Fourier heat flux of both phases.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(128) k_viscous(Params P, State s, Mesh m)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ne = m.ne;
+    if (i >= ne || m.iblank[i] != 1) return;
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    const int slot[8] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP};
+    double acc[8];
+    for (int v = 0; v < 8; v++) acc[v] = 0.0;
+    int nci[4];
+    for (int k = 0; k < 4; k++) nci[k] = m.nc[(size_t)k * ne + i];
+    for (int f = 0; f < 4; f++) {
+        const double scx = m.sc[(size_t)(3 * f) * ne + i], scy = m.sc[(size_t)(3 * f + 1) * ne + i],
+                     scz = m.sc[(size_t)(3 * f + 2) * ne + i];
+        const double xf = m.fc3[(size_t)(3 * f) * ne + i], yf = m.fc3[(size_t)(3 * f + 1) * ne + i],
+                     zf = m.fc3[(size_t)(3 * f + 2) * ne + i];
+        const int neigh = nci[f];
+        double fv[8];
+        for (int v = 0; v < 8; v++) fv[v] = (QV(slot[v], i) + QV(slot[v], neigh)) * 0.5;
+        const double phigf = (QV(Q_PHIG, i) + QV(Q_PHIG, neigh)) * 0.5;
+        const double phipf = (QV(Q_PHIP, i) + QV(Q_PHIP, neigh)) * 0.5;
+        const double tgf = fv[3];
+        const double amug = P.as * sqrt(tgf) / (1.0 + P.ts / tgf);
+        const double akg = amug * P.cpg / P.prg;
+        const double amup = 0.001;
+        const double akp = amup * P.cpp / P.prp;
+        int mm[8];
+        int nm;
+        mm[0] = nci[0]; mm[1] = nci[1]; mm[2] = nci[2]; mm[3] = nci[3];
+        if (neigh >= ne) {
+            mm[4] = i;
+            nm = 5;
+        } else {
+            for (int k = 0; k < 4; k++) mm[4 + k] = m.nc[(size_t)k * ne + neigh];
+            nm = 8;
+        }
+        double dx[8], dy[8], dz[8];
+        for (int k = 0; k < nm; k++) {
+            dx[k] = m.xc[mm[k]] - xf;
+            dy[k] = m.yc[mm[k]] - yf;
+            dz[k] = m.zc[mm[k]] - zf;
+        }
+        double a11 = dx[0] * dx[0], a22 = dy[0] * dy[0], a33 = dz[0] * dz[0];
+        double a12 = dx[0] * dy[0], a13 = dx[0] * dz[0], a23 = dy[0] * dz[0];
+        for (int k = 1; k < nm; k++) {
+            a11 = a11 + dx[k] * dx[k];
+            a22 = a22 + dy[k] * dy[k];
+            a33 = a33 + dz[k] * dz[k];
+            a12 = a12 + dx[k] * dy[k];
+            a13 = a13 + dx[k] * dz[k];
+            a23 = a23 + dy[k] * dz[k];
+        }
+        const double dnr = det3(a11, a12, a13, a12, a22, a23, a13, a23, a33);
+        double gr[8][3];
+        for (int v = 0; v < 8; v++) {
+            double dv = QV(slot[v], mm[0]) - fv[v];
+            double b1 = dv * dx[0], b2 = dv * dy[0], b3 = dv * dz[0];
+            for (int k = 1; k < nm; k++) {
+                dv = QV(slot[v], mm[k]) - fv[v];
+                b1 = b1 + dv * dx[k];
+                b2 = b2 + dv * dy[k];
+                b3 = b3 + dv * dz[k];
+            }
+            gr[v][0] = det3(b1, b2, b3, a12, a22, a23, a13, a23, a33) / dnr;
+            gr[v][1] = det3(a11, a12, a13, b1, b2, b3, a13, a23, a33) / dnr;
+            gr[v][2] = det3(a11, a12, a13, a12, a22, a23, b1, b2, b3) / dnr;
+        }
+        // gas (v = 0..3) and liquid (v = 4..7) stress + heat flux
+        for (int ph = 0; ph < 2; ph++) {
+            const int o = 4 * ph;
+            const double mu = ph ? amup : amug, ak = ph ? akp : akg;
+            const double uf = fv[o], vf = fv[o + 1], wf = fv[o + 2];
+            const double dux = gr[o][0], duy = gr[o][1], duz = gr[o][2];
+            const double dvx = gr[o + 1][0], dvy = gr[o + 1][1], dvz = gr[o + 1][2];
+            const double dwx = gr[o + 2][0], dwy = gr[o + 2][1], dwz = gr[o + 2][2];
+            const double dtx = gr[o + 3][0], dty = gr[o + 3][1], dtz = gr[o + 3][2];
+            const double delv = dux + dvy + dwz;
+            const double txx = mu * (2.0 * dux - P.c23 * delv);
+            const double tyy = mu * (2.0 * dvy - P.c23 * delv);
+            const double tzz = mu * (2.0 * dwz - P.c23 * delv);
+            const double txy = mu * (dvx + duy);
+            const double txz = mu * (dwx + duz);
+            const double tyz = mu * (dvz + dwy);
+            const double qxx = uf * txx + vf * txy + wf * txz + ak * dtx;
+            const double qyy = uf * txy + vf * tyy + wf * tyz + ak * dty;
+            const double qzz = uf * txz + vf * tyz + wf * tzz + ak * dtz;
+            const double phf = ph ? phipf : phigf;
+            acc[o + 0] = acc[o + 0] + phf * (txx * scx + txy * scy + txz * scz);
+            acc[o + 1] = acc[o + 1] + phf * (txy * scx + tyy * scy + tyz * scz);
+            acc[o + 2] = acc[o + 2] + phf * (txz * scx + tyz * scy + tzz * scz);
+            acc[o + 3] = acc[o + 3] + phf * (qxx * scx + qyy * scy + qzz * scz);
+        }
+    }
+    for (int v = 0; v < 8; v++) s.vis[(size_t)v * ne + i] = acc[v];
+}
+
+// ---------------------------------------------------------------------------
+// cell gradients (sr_derivative.F90:45-322, inverse-squared-distance LSQ over
+// the 4 face neighbours, Cramer's rule) fused with the Venkatakrishnan
+// limiter (sr_VK.F90:32-274), which only needs this cell's gradients.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(128) k_grad_limiter(State s, Mesh m)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ne = m.ne;
+    if (i >= ne || m.iblank[i] != 1) return;
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    int nb[4];
+    for (int k = 0; k < 4; k++) nb[k] = m.nc[(size_t)k * ne + i];
+    const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i];
+    double dx[4], dy[4], dz[4], wt[4];
+    for (int k = 0; k < 4; k++) {
+        dx[k] = m.xc[nb[k]] - xi;
+        dy[k] = m.yc[nb[k]] - yi;
+        dz[k] = m.zc[nb[k]] - zi;
+        wt[k] = dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k];
+    }
+    const double a1 = dx[0] * dx[0] / wt[0] + dx[1] * dx[1] / wt[1] + dx[2] * dx[2] / wt[2] + dx[3] * dx[3] / wt[3];
+    const double a2 = dy[0] * dy[0] / wt[0] + dy[1] * dy[1] / wt[1] + dy[2] * dy[2] / wt[2] + dy[3] * dy[3] / wt[3];
+    const double a3 = dz[0] * dz[0] / wt[0] + dz[1] * dz[1] / wt[1] + dz[2] * dz[2] / wt[2] + dz[3] * dz[3] / wt[3];
+    const double b1 = dx[0] * dy[0] / wt[0] + dx[1] * dy[1] / wt[1] + dx[2] * dy[2] / wt[2] + dx[3] * dy[3] / wt[3];
+    const double b2 = dx[0] * dz[0] / wt[0] + dx[1] * dz[1] / wt[1] + dx[2] * dz[2] / wt[2] + dx[3] * dz[3] / wt[3];
+    const double b3 = dz[0] * dy[0] / wt[0] + dz[1] * dy[1] / wt[1] + dz[2] * dy[2] / wt[2] + dz[3] * dy[3] / wt[3];
+    const double dnr = det3(a1, b1, b2, b1, a2, b3, b2, b3, a3);
+    // 13 scalars; the first 11 feed the face reconstruction and are stored,
+    // rog/rop (and pg/pp) feed the limiter below
+    const int slot[13] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP, Q_PG, Q_PP, Q_PHIG, Q_ROG, Q_ROP};
+    double glim[4][3];  // pg, pp, rog, rop
+    for (int sI = 0; sI < 13; sI++) {
+        const int sl = slot[sI];
+        const double qi = QV(sl, i);
+        double dq[4];
+        for (int k = 0; k < 4; k++) dq[k] = QV(sl, nb[k]) - qi;
+        const double c1 = dq[0] * dx[0] / wt[0] + dq[1] * dx[1] / wt[1] + dq[2] * dx[2] / wt[2] + dq[3] * dx[3] / wt[3];
+        double c2;
+        if (sl == Q_TP)  // sr_derivative.F90:225, last term without /wt4
+            c2 = dq[0] * dy[0] / wt[0] + dq[1] * dy[1] / wt[1] + dq[2] * dy[2] / wt[2] + dq[3] * dy[3];
+        else
+            c2 = dq[0] * dy[0] / wt[0] + dq[1] * dy[1] / wt[1] + dq[2] * dy[2] / wt[2] + dq[3] * dy[3] / wt[3];
+        const double c3 = dq[0] * dz[0] / wt[0] + dq[1] * dz[1] / wt[1] + dq[2] * dz[2] / wt[2] + dq[3] * dz[3] / wt[3];
+        const double gx = det3(c1, c2, c3, b1, a2, b3, b2, b3, a3) / dnr;
+        const double gy = det3(a1, b1, b2, c1, c2, c3, b2, b3, a3) / dnr;
+        const double gz = det3(a1, b1, b2, b1, a2, b3, c1, c2, c3) / dnr;
+        if (sI < 11) {
+            s.grad[(size_t)(3 * sI + 0) * ne + i] = gx;
+            s.grad[(size_t)(3 * sI + 1) * ne + i] = gy;
+            s.grad[(size_t)(3 * sI + 2) * ne + i] = gz;
+        }
+        const int li = (sl == Q_PG) ? 0 : (sl == Q_PP) ? 1 : (sl == Q_ROG) ? 2 : (sl == Q_ROP) ? 3 : -1;
+        if (li >= 0) { glim[li][0] = gx; glim[li][1] = gy; glim[li][2] = gz; }
+    }
+    // limiter (sr_VK.F90:34-272)
+    const double epssq = m.epssq[i];
+    double r0[4][3];
+    for (int f = 0; f < 4; f++) {
+        r0[f][0] = m.fc0[(size_t)(3 * f) * ne + i] - xi;
+        r0[f][1] = m.fc0[(size_t)(3 * f + 1) * ne + i] - yi;
+        r0[f][2] = m.fc0[(size_t)(3 * f + 2) * ne + i] - zi;
+    }
+    const int lq[4] = {Q_PG, Q_PP, Q_ROG, Q_ROP};
+    double ph = 0.0;
+    for (int v = 0; v < 4; v++) {
+        const double qi = QV(lq[v], i);
+        double vmax = qi, vmin = qi;
+        for (int k = 0; k < 4; k++) {
+            const double qn = QV(lq[v], nb[k]);
+            vmax = dmax(vmax, qn);
+            vmin = dmin(vmin, qn);
+        }
+        for (int f = 0; f < 4; f++) {
+            const double delf = r0[f][0] * glim[v][0] + r0[f][1] * glim[v][1] + r0[f][2] * glim[v][2];
+            double pv;
+            if (delf > 0.0)
+                pv = fvenkat(vmax - qi, delf, epssq);
+            else if (delf < 0.0)
+                pv = fvenkat(vmin - qi, delf, epssq);
+            else
+                pv = 1.0;
+            ph = (v == 0 && f == 0) ? pv : dmin(ph, pv);
+        }
+    }
+    s.phi[i] = ph;
+}
+
+// ---------------------------------------------------------------------------
+// face loop + interphase source + optional RK stage update
+// ---------------------------------------------------------------------------
+struct FState {
+    double rog, ug, vg, wg, tg, phig, rop, up, vp, wp, tp, pg, pp, phip;
+};
+
+__device__ __forceinline__ void load_fstate(const double *q, size_t nt, int i, FState &s)
+{
+    s.rog = QV(Q_ROG, i); s.ug = QV(Q_UG, i); s.vg = QV(Q_VG, i); s.wg = QV(Q_WG, i);
+    s.tg = QV(Q_TG, i); s.phig = QV(Q_PHIG, i); s.rop = QV(Q_ROP, i); s.up = QV(Q_UP, i);
+    s.vp = QV(Q_VP, i); s.wp = QV(Q_WP, i); s.tp = QV(Q_TP, i); s.pg = QV(Q_PG, i); s.pp = QV(Q_PP, i);
+}
+
+// sr_hllcbatten.F90:228-318
+__device__ __forceinline__ void clip_blend(const Params &P, FState &s)
+{
+    if (s.phig < P.epslnmin) {
+        s.phig = P.epslnmin; s.phip = 1.0 - s.phig;
+        s.ug = s.up; s.vg = s.vp; s.wg = s.wp; s.tg = s.tp; s.pg = s.pp;
+    }
+    if (s.phip < P.epslnmin) {
+        s.phip = P.epslnmin; s.phig = 1.0 - s.phip;
+        s.up = s.ug; s.vp = s.vg; s.wp = s.wg; s.tp = s.tg; s.pp = s.pg;
+    }
+    if (s.phig >= P.epslnmin && s.phig <= P.epslnmax) {
+        const double psig = (s.phig - P.epslnmin) / (P.epslnmax - P.epslnmin);
+        const double fg = -psig * psig * (2.0 * psig - 3.0);
+        s.ug = fg * s.ug + (1.0 - fg) * s.up;
+        s.vg = fg * s.vg + (1.0 - fg) * s.vp;
+        s.wg = fg * s.wg + (1.0 - fg) * s.wp;
+        s.tg = fg * s.tg + (1.0 - fg) * s.tp;
+        s.pg = fg * s.pg + (1.0 - fg) * s.pp;
+    }
+    if (s.phip >= P.epslnmin && s.phip <= P.epslnmax) {
+        const double psig = (s.phip - P.epslnmin) / (P.epslnmax - P.epslnmin);
+        const double fg = -psig * psig * (2.0 * psig - 3.0);
+        s.up = fg * s.up + (1.0 - fg) * s.ug;
+        s.vp = fg * s.vp + (1.0 - fg) * s.vg;
+        s.wp = fg * s.wp + (1.0 - fg) * s.wg;
+        s.tp = fg * s.tp + (1.0 - fg) * s.tg;
+        s.pp = fg * s.pp + (1.0 - fg) * s.pg;
+    }
+    s.rog = s.pg / (P.rrg * s.tg);
+    s.rop = P.gamap * (s.pp + P.pinf) / ((P.gamap - 1.0) * P.cpp * s.tp);
+}
+
+// One phase of the HLLC-Batten flux (sr_hllcbatten.F90:338-529): each phase
+// has its own wave speeds and contact.  g = gas (energy from R T/(gamma-1)),
+// otherwise liquid (stiffened gas: cp T/gamma + p_inf/rho).
+template <bool GAS>
+__device__ __forceinline__ void hllc_phase(const Params &P, double phL, double roL, double uL, double vL,
+                                           double wL, double tL, double pL, double phR, double roR, double uR,
+                                           double vR, double wR, double tR, double pR, double enx, double eny,
+                                           double enz, double area, double *F)
+{
+    double EL, ER, aL, aR;
+    if (GAS) {
+        EL = P.rrg * tL / (P.gamag - 1.0) + 0.5 * (uL * uL + vL * vL + wL * wL);
+        ER = P.rrg * tR / (P.gamag - 1.0) + 0.5 * (uR * uR + vR * vR + wR * wR);
+        aL = sqrt(P.gamag * pL / roL);
+        aR = sqrt(P.gamag * pR / roR);
+    } else {
+        EL = (P.cpp * tL) / P.gamap + P.pinf / roL + 0.5 * (uL * uL + vL * vL + wL * wL);
+        ER = (P.cpp * tR) / P.gamap + P.pinf / roR + 0.5 * (uR * uR + vR * vR + wR * wR);
+        aL = sqrt(P.gamap * (pL + P.pinf) / roL);
+        aR = sqrt(P.gamap * (pR + P.pinf) / roR);
+    }
+    const double HL = EL + pL / roL, HR = ER + pR / roR;
+    const double qL = uL * enx + vL * eny + wL * enz;
+    const double qR = uR * enx + vR * eny + wR * enz;
+    const double SL = dmin(dmin(qL - aL, qR - aR), F1E8);
+    const double SR = dmax(dmax(qL + aL, qR + aR), F1E8);
+    const double snr = pR - pL + roL * qL * (SL - qL) - roR * qR * (SR - qR);
+    const double sdr = roL * (SL - qL) - roR * (SR - qR);
+    const double SS = snr / (sdr + F1E8);
+    const double pst = dmax(pL + roL * (qL - SL) * (qL - SS), F1E6 * dmin(pL, pR));
+    const double mL = phL * roL, mR = phR * roR;
+    if (SL > 0.0) {
+        F[0] = mL * qL * area;
+        F[1] = (mL * qL * uL + phL * pL * enx) * area;
+        F[2] = (mL * qL * vL + phL * pL * eny) * area;
+        F[3] = (mL * qL * wL + phL * pL * enz) * area;
+        F[4] = mL * qL * HL * area;
+    } else if (SL <= 0.0 && SS > 0.0) {
+        const double dn = (SS - SL) + F1E6;
+        const double c1 = mL, c2 = mL * uL, c3 = mL * vL, c4 = mL * wL, c5 = mL * EL;
+        const double s1 = (qL - SL) * c1 / dn;
+        const double s2 = ((qL - SL) * c2 + phL * (pL * enx - pst * enx)) / dn;
+        const double s3 = ((qL - SL) * c3 + phL * (pL * eny - pst * eny)) / dn;
+        const double s4 = ((qL - SL) * c4 + phL * (pL * enz - pst * enz)) / dn;
+        const double s5 = ((qL - SL) * c5 + phL * (pL * qL - pst * SS)) / dn;
+        F[0] = (mL * qL + SL * (s1 - c1)) * area;
+        F[1] = (mL * qL * uL + phL * pL * enx + SL * (s2 - c2)) * area;
+        F[2] = (mL * qL * vL + phL * pL * eny + SL * (s3 - c3)) * area;
+        F[3] = (mL * qL * wL + phL * pL * enz + SL * (s4 - c4)) * area;
+        F[4] = (mL * qL * HL + SL * (s5 - c5)) * area;
+    } else if (SS <= 0.0 && SR >= 0.0) {
+        const double dn = (SS - SR) + F1E6;
+        const double c1 = mR, c2 = mR * uR, c3 = mR * vR, c4 = mR * wR, c5 = mR * ER;
+        const double s1 = (qR - SR) * c1 / dn;
+        const double s2 = ((qR - SR) * c2 + phR * (pR * enx - pst * enx)) / dn;
+        const double s3 = ((qR - SR) * c3 + phR * (pR * eny - pst * eny)) / dn;
+        const double s4 = ((qR - SR) * c4 + phR * (pR * enz - pst * enz)) / dn;
+        const double s5 = ((qR - SR) * c5 + phR * (pR * qR - pst * SS)) / dn;
+        F[0] = (mR * qR + SR * (s1 - c1)) * area;
+        F[1] = (mR * qR * uR + phR * pR * enx + SR * (s2 - c2)) * area;
+        F[2] = (mR * qR * vR + phR * pR * eny + SR * (s3 - c3)) * area;
+        F[3] = (mR * qR * wR + phR * pR * enz + SR * (s4 - c4)) * area;
+        F[4] = (mR * qR * HR + SR * (s5 - c5)) * area;
+    } else if (SR < 0.0) {
+        F[0] = mR * qR * area;
+        F[1] = (mR * qR * uR + phR * pR * enx) * area;
+        F[2] = (mR * qR * vR + phR * pR * eny) * area;
+        F[3] = (mR * qR * wR + phR * pR * enz) * area;
+        F[4] = mR * qR * HR * area;
+    }
+}
+
+// HLLC-Zien flux (sr_hllczien.F90:327-487): mixture wave speeds and one
+// contact shared by both phases.
+__device__ __forceinline__ void hllc_zien(const Params &P, const FState &L, const FState &R, double enx,
+                                          double eny, double enz, double area, double *F)
+{
+    const double EGL = P.rrg * L.tg / (P.gamag - 1.0) + 0.5 * (L.ug * L.ug + L.vg * L.vg + L.wg * L.wg);
+    const double HGL = EGL + L.pg / L.rog;
+    const double QGL = L.ug * enx + L.vg * eny + L.wg * enz;
+    const double AGL = sqrt(P.gamag * L.pg / L.rog);
+    const double EGR = P.rrg * R.tg / (P.gamag - 1.0) + 0.5 * (R.ug * R.ug + R.vg * R.vg + R.wg * R.wg);
+    const double HGR = EGR + R.pg / R.rog;
+    const double QGR = R.ug * enx + R.vg * eny + R.wg * enz;
+    const double AGR = sqrt(P.gamag * R.pg / R.rog);
+    const double EPL = (P.cpp * L.tp) / P.gamap + P.pinf / L.rop + 0.5 * (L.up * L.up + L.vp * L.vp + L.wp * L.wp);
+    const double HPL = EPL + L.pp / L.rop;
+    const double QPL = L.up * enx + L.vp * eny + L.wp * enz;
+    const double APL = sqrt(P.gamap * (L.pp + P.pinf) / L.rop);
+    const double EPR = (P.cpp * R.tp) / P.gamap + P.pinf / R.rop + 0.5 * (R.up * R.up + R.vp * R.vp + R.wp * R.wp);
+    const double HPR = EPR + R.pp / R.rop;
+    const double QPR = R.up * enx + R.vp * eny + R.wp * enz;
+    const double APR = sqrt(P.gamap * (R.pp + P.pinf) / R.rop);
+    const double SL = dmin(dmin(dmin(dmin(QGL - AGL, QPL - APL), QGR - AGR), QPR - APR), F1E8);
+    const double SR = dmax(dmax(dmax(dmax(QGL + AGL, QPL + APL), QGR + AGR), QPR + APR), F1E8);
+    const double ROL = L.phig * L.rog + L.phip * L.rop;
+    const double ROQL = L.phig * L.rog * QGL + L.phip * L.rop * QPL;
+    const double QL = ROQL / ROL;
+    const double PRL = L.phig * L.pg + L.phip * L.pp;
+    const double ROR = R.phig * R.rog + R.phip * R.rop;
+    const double ROQR = R.phig * R.rog * QGR + R.phip * R.rop * QPR;
+    const double QR = ROQR / ROR;
+    const double PRR = R.phig * R.pg + R.phip * R.pp;
+    const double SSTAR = (PRR - PRL + ROQL * (SL - QL) - ROQR * (SR - QR)) / (ROL * (SL - QL) - ROR * (SR - QR) + F1E8);
+    const double PGSTAR = dmax(L.pg + L.rog * (QGL - SL) * (QGL - SSTAR), F1E6 * dmin(L.pg, R.pg));
+    const double PPSTAR = dmax(L.pp + L.rop * (QPL - SL) * (QPL - SSTAR), F1E6 * dmin(L.pp, R.pp));
+    const double mgL = L.phig * L.rog, mpL = L.phip * L.rop, mgR = R.phig * R.rog, mpR = R.phip * R.rop;
+    if (SL >= 0.0) {
+        F[0] = mgL * QGL * area;
+        F[1] = (mgL * QGL * L.ug + L.phig * L.pg * enx) * area;
+        F[2] = (mgL * QGL * L.vg + L.phig * L.pg * eny) * area;
+        F[3] = (mgL * QGL * L.wg + L.phig * L.pg * enz) * area;
+        F[4] = mgL * QGL * HGL * area;
+        F[5] = mpL * QPL * area;
+        F[6] = (mpL * QPL * L.up + L.phip * L.pp * enx) * area;
+        F[7] = (mpL * QPL * L.vp + L.phip * L.pp * eny) * area;
+        F[8] = (mpL * QPL * L.wp + L.phip * L.pp * enz) * area;
+        F[9] = mpL * QPL * HPL * area;
+    } else if (SL <= 0.0 && SSTAR > 0.0) {
+        const double dn = (SSTAR - SL) + F1E6;
+        const double cg[5] = {mgL, mgL * L.ug, mgL * L.vg, mgL * L.wg, mgL * EGL};
+        const double cp[5] = {mpL, mpL * L.up, mpL * L.vp, mpL * L.wp, mpL * EPL};
+        const double sg[5] = {(QGL - SL) * cg[0] / dn,
+                              ((QGL - SL) * cg[1] + L.phig * (L.pg * enx - PGSTAR * enx)) / dn,
+                              ((QGL - SL) * cg[2] + L.phig * (L.pg * eny - PGSTAR * eny)) / dn,
+                              ((QGL - SL) * cg[3] + L.phig * (L.pg * enz - PGSTAR * enz)) / dn,
+                              ((QGL - SL) * cg[4] + L.phig * (L.pg * QGL - PGSTAR * SSTAR)) / dn};
+        const double sp[5] = {(QPL - SL) * cp[0] / dn,
+                              ((QPL - SL) * cp[1] + L.phip * (L.pp * enx - PPSTAR * enx)) / dn,
+                              ((QPL - SL) * cp[2] + L.phip * (L.pp * eny - PPSTAR * eny)) / dn,
+                              ((QPL - SL) * cp[3] + L.phip * (L.pp * enz - PPSTAR * enz)) / dn,
+                              ((QPL - SL) * cp[4] + L.phip * (L.pp * QPL - PPSTAR * SSTAR)) / dn};
+        F[0] = (mgL * QGL + SL * (sg[0] - cg[0])) * area;
+        F[1] = (mgL * QGL * L.ug + L.phig * L.pg * enx + SL * (sg[1] - cg[1])) * area;
+        F[2] = (mgL * QGL * L.vg + L.phig * L.pg * eny + SL * (sg[2] - cg[2])) * area;
+        F[3] = (mgL * QGL * L.wg + L.phig * L.pg * enz + SL * (sg[3] - cg[3])) * area;
+        F[4] = (mgL * QGL * HGL + SL * (sg[4] - cg[4])) * area;
+        F[5] = (mpL * QPL + SL * (sp[0] - cp[0])) * area;
+        F[6] = (mpL * QPL * L.up + L.phip * L.pp * enx + SL * (sp[1] - cp[1])) * area;
+        F[7] = (mpL * QPL * L.vp + L.phip * L.pp * eny + SL * (sp[2] - cp[2])) * area;
+        F[8] = (mpL * QPL * L.wp + L.phip * L.pp * enz + SL * (sp[3] - cp[3])) * area;
+        F[9] = (mpL * QPL * HPL + SL * (sp[4] - cp[4])) * area;
+    } else if (SSTAR <= 0.0 && SR >= 0.0) {
+        const double dn = (SSTAR - SR) + F1E6;
+        const double cg[5] = {mgR, mgR * R.ug, mgR * R.vg, mgR * R.wg, mgR * EGR};
+        const double cp[5] = {mpR, mpR * R.up, mpR * R.vp, mpR * R.wp, mpR * EPR};
+        const double sg[5] = {(QGR - SR) * cg[0] / dn,
+                              ((QGR - SR) * cg[1] + R.phig * (R.pg * enx - PGSTAR * enx)) / dn,
+                              ((QGR - SR) * cg[2] + R.phig * (R.pg * eny - PGSTAR * eny)) / dn,
+                              ((QGR - SR) * cg[3] + R.phig * (R.pg * enz - PGSTAR * enz)) / dn,
+                              ((QGR - SR) * cg[4] + R.phig * (R.pg * QGR - PGSTAR * SSTAR)) / dn};
+        const double sp[5] = {(QPR - SR) * cp[0] / dn,
+                              ((QPR - SR) * cp[1] + R.phip * (R.pp * enx - PPSTAR * enx)) / dn,
+                              ((QPR - SR) * cp[2] + R.phip * (R.pp * eny - PPSTAR * eny)) / dn,
+                              ((QPR - SR) * cp[3] + R.phip * (R.pp * enz - PPSTAR * enz)) / dn,
+                              ((QPR - SR) * cp[4] + R.phip * (R.pp * QPR - PPSTAR * SSTAR)) / dn};
+        F[0] = (mgR * QGR + SR * (sg[0] - cg[0])) * area;
+        F[1] = (mgR * QGR * R.ug + R.phig * R.pg * enx + SR * (sg[1] - cg[1])) * area;
+        F[2] = (mgR * QGR * R.vg + R.phig * R.pg * eny + SR * (sg[2] - cg[2])) * area;
+        F[3] = (mgR * QGR * R.wg + R.phig * R.pg * enz + SR * (sg[3] - cg[3])) * area;
+        F[4] = (mgR * QGR * HGR + SR * (sg[4] - cg[4])) * area;
+        F[5] = (mpR * QPR + SR * (sp[0] - cp[0])) * area;
+        F[6] = (mpR * QPR * R.up + R.phip * R.pp * enx + SR * (sp[1] - cp[1])) * area;
+        F[7] = (mpR * QPR * R.vp + R.phip * R.pp * eny + SR * (sp[2] - cp[2])) * area;
+        F[8] = (mpR * QPR * R.wp + R.phip * R.pp * enz + SR * (sp[3] - cp[3])) * area;
+        F[9] = (mpR * QPR * HPR + SR * (sp[4] - cp[4])) * area;
+    } else if (SR <= 0.0) {
+        F[0] = mgR * QGR * area;
+        F[1] = (mgR * QGR * R.ug + R.phig * R.pg * enx) * area;
+        F[2] = (mgR * QGR * R.vg + R.phig * R.pg * eny) * area;
+        F[3] = (mgR * QGR * R.wg + R.phig * R.pg * enz) * area;
+        F[4] = mgR * QGR * HGR * area;
+        F[5] = mpR * QPR * area;
+        F[6] = (mpR * QPR * R.up + R.phip * R.pp * enx) * area;
+        F[7] = (mpR * QPR * R.vp + R.phip * R.pp * eny) * area;
+        F[8] = (mpR * QPR * R.wp + R.phip * R.pp * enz) * area;
+        F[9] = mpR * QPR * HPR * area;
+    }
+}
+
+#define GRAD(s, d, c) s_grad[(size_t)(3 * (s) + (d)) * ne + (c)]
+
+// stage >= 1: apply the RK stage update (sr_RK.F90) from the fused residual;
+// stage == 0: store the residual in rhs (per-routine checks).
+template <int FLUX>
+__global__ void __launch_bounds__(128) k_flux(Params P, State s, Mesh m, int stage)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ne = m.ne;
+    if (i >= ne || m.iblank[i] != 1) return;
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    const double *s_grad = s.grad;
+    double rhs[10];
+    for (int k = 0; k < 10; k++) rhs[k] = 0.0;
+    const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i];
+    const double phii = s.phi[i];
+    for (int f = 0; f < 4; f++) {
+        const int neigh = m.nc[(size_t)f * ne + i];
+        const double scx = m.sc[(size_t)(3 * f) * ne + i], scy = m.sc[(size_t)(3 * f + 1) * ne + i],
+                     scz = m.sc[(size_t)(3 * f + 2) * ne + i];
+        const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
+                     zf = m.fc0[(size_t)(3 * f + 2) * ne + i];
+        const double area = sqrt(scx * scx + scy * scy + scz * scz);
+        const double enx = scx / area, eny = scy / area, enz = scz / area;
+        const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
+        FState L, R;
+        load_fstate(q, nt, i, L);
+        load_fstate(q, nt, neigh, R);
+        if (P.myorder == 2 && neigh < ne) {
+            // MUSCL reconstruction with one limiter per face pair
+            // (sr_hllcbatten.F90:166-224)
+            const double phimin = dmin(phii, s.phi[neigh]);
+            const double xf0 = xf - xi, yf0 = yf - yi, zf0 = zf - zi;
+            const double xfn = xf - m.xc[neigh], yfn = yf - m.yc[neigh], zfn = zf - m.zc[neigh];
+#define REC(fld, sl)                                                                                  \
+    L.fld = L.fld + phimin * (xf0 * GRAD(sl, 0, i) + yf0 * GRAD(sl, 1, i) + zf0 * GRAD(sl, 2, i));     \
+    R.fld = R.fld + phimin * (xfn * GRAD(sl, 0, neigh) + yfn * GRAD(sl, 1, neigh) + zfn * GRAD(sl, 2, neigh));
+            REC(phig, G_PHIG) REC(ug, G_UG) REC(vg, G_VG) REC(wg, G_WG) REC(tg, G_TG) REC(pg, G_PG)
+            REC(up, G_UP) REC(vp, G_VP) REC(wp, G_WP) REC(tp, G_TP) REC(pp, G_PP)
+#undef REC
+        }
+        L.phip = 1.0 - L.phig;
+        R.phip = 1.0 - R.phig;
+        clip_blend(P, L);
+        clip_blend(P, R);
+        if (ntype == 5) {
+            // slip wall: pressure force only (sr_hllcbatten.F90:324-336)
+            rhs[1] = rhs[1] + L.phig * L.pg * enx * area;
+            rhs[2] = rhs[2] + L.phig * L.pg * eny * area;
+            rhs[3] = rhs[3] + L.phig * L.pg * enz * area;
+            rhs[6] = rhs[6] + L.phip * L.pp * enx * area;
+            rhs[7] = rhs[7] + L.phip * L.pp * eny * area;
+            rhs[8] = rhs[8] + L.phip * L.pp * enz * area;
+        } else {
+            double F[10];
+            if (FLUX == 1) {
+                hllc_zien(P, L, R, enx, eny, enz, area, F);
+            } else {
+                hllc_phase<true>(P, L.phig, L.rog, L.ug, L.vg, L.wg, L.tg, L.pg, R.phig, R.rog, R.ug, R.vg,
+                                 R.wg, R.tg, R.pg, enx, eny, enz, area, F);
+                hllc_phase<false>(P, L.phip, L.rop, L.up, L.vp, L.wp, L.tp, L.pp, R.phip, R.rop, R.up, R.vp,
+                                  R.wp, R.tp, R.pp, enx, eny, enz, area, F + 5);
+            }
+            for (int k = 0; k < 10; k++) rhs[k] = rhs[k] + F[k];
+        }
+    }
+    // interphase source (sr_source.F90:46-155)
+    {
+        const double phig = QV(Q_PHIG, i), phip = QV(Q_PHIP, i), rog = QV(Q_ROG, i), rop = QV(Q_ROP, i);
+        const double ug = QV(Q_UG, i), vg = QV(Q_VG, i), wg = QV(Q_WG, i), up = QV(Q_UP, i), vp = QV(Q_VP, i),
+                     wp = QV(Q_WP, i), tg = QV(Q_TG, i), tp = QV(Q_TP, i), pg = QV(Q_PG, i);
+        const double voli = m.vol[i], dl = m.dl[i];
+        const double amug = P.as * sqrt(tg) / (1.0 + P.ts / tg);
+        const double du = ug - up, dv = vg - vp, dw = wg - wp;
+        const double drag = 0.44 * phig * (phig - 1.0) * rog * sqrt(du * du + dv * dv + dw * dw) * dl;
+        const double fdgx = drag * du, fdgy = drag * dv, fdgz = drag * dw;
+        const double qsource = 6.0 * phip * amug * P.cpg * 2.0 * (tp - tg) / (P.prg * (dl * dl));
+        const double delpstar = 2.0 * (phig * rog * phip * rop) / (phig * rop + phip * rog) * (du * du + dv * dv + dw * dw);
+        const double pint = pg - dmin(delpstar, 0.01 * pg);
+        const double den = phig * rog + (1.0 - phig) * rop;
+        const double uint = (phig * rog * ug + (1.0 - phig) * rop * up) / den;
+        const double vint = (phig * rog * vg + (1.0 - phig) * rop * vp) / den;
+        const double wint = (phig * rog * wg + (1.0 - phig) * rop * wp) / den;
+        double dx[4], dy[4], dz[4], wt[4], dph[4];
+        for (int k = 0; k < 4; k++) {
+            const int nb = m.nc[(size_t)k * ne + i];
+            dx[k] = m.xc[nb] - xi;
+            dy[k] = m.yc[nb] - yi;
+            dz[k] = m.zc[nb] - zi;
+            dph[k] = QV(Q_PHIG, nb) - phig;
+            wt[k] = sqrt(dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k]);
+        }
+        const double a1 = dx[0] * dx[0] / wt[0] + dx[1] * dx[1] / wt[1] + dx[2] * dx[2] / wt[2] + dx[3] * dx[3] / wt[3];
+        const double a2 = dy[0] * dy[0] / wt[0] + dy[1] * dy[1] / wt[1] + dy[2] * dy[2] / wt[2] + dy[3] * dy[3] / wt[3];
+        const double a3 = dz[0] * dz[0] / wt[0] + dz[1] * dz[1] / wt[1] + dz[2] * dz[2] / wt[2] + dz[3] * dz[3] / wt[3];
+        const double b1 = dx[0] * dy[0] / wt[0] + dx[1] * dy[1] / wt[1] + dx[2] * dy[2] / wt[2] + dx[3] * dy[3] / wt[3];
+        const double b2 = dx[0] * dz[0] / wt[0] + dx[1] * dz[1] / wt[1] + dx[2] * dz[2] / wt[2] + dx[3] * dz[3] / wt[3];
+        const double b3 = dy[0] * dz[0] / wt[0] + dy[1] * dz[1] / wt[1] + dy[2] * dz[2] / wt[2] + dy[3] * dz[3] / wt[3];
+        const double c1 = dph[0] * dx[0] / wt[0] + dph[1] * dx[1] / wt[1] + dph[2] * dx[2] / wt[2] + dph[3] * dx[3] / wt[3];
+        const double c2 = dph[0] * dy[0] / wt[0] + dph[1] * dy[1] / wt[1] + dph[2] * dy[2] / wt[2] + dph[3] * dy[3] / wt[3];
+        const double c3 = dph[0] * dz[0] / wt[0] + dph[1] * dz[1] / wt[1] + dph[2] * dz[2] / wt[2] + dph[3] * dz[3] / wt[3];
+        const double dnr = det3(a1, b1, b2, b1, a2, b3, b2, b3, a3);
+        const double dphx = det3(c1, c2, c3, b1, a2, b3, b2, b3, a3) / dnr;
+        const double dphy = det3(a1, b1, b2, c1, c2, c3, b2, b3, a3) / dnr;
+        const double dphz = det3(a1, b1, b2, b1, a2, b3, c1, c2, c3) / dnr;
+        const double work = pint * (uint * dphx + vint * dphy + wint * dphz) + uint * fdgx + vint * fdgy +
+                             wint * fdgz + qsource;
+        const double *vis = s.vis;
+        rhs[1] = rhs[1] - (fdgx + pint * dphx + phig * rog * P.gravity) * voli - vis[(size_t)0 * ne + i];
+        rhs[2] = rhs[2] - (fdgy + pint * dphy) * voli - vis[(size_t)1 * ne + i];
+        rhs[3] = rhs[3] - (fdgz + pint * dphz) * voli - vis[(size_t)2 * ne + i];
+        rhs[4] = rhs[4] - work * voli - vis[(size_t)3 * ne + i];
+        rhs[6] = rhs[6] + (fdgx + pint * dphx - phip * rop * P.gravity) * voli - vis[(size_t)4 * ne + i];
+        rhs[7] = rhs[7] + (fdgy + pint * dphy) * voli - vis[(size_t)5 * ne + i];
+        rhs[8] = rhs[8] + (fdgz + pint * dphz) * voli - vis[(size_t)6 * ne + i];
+        rhs[9] = rhs[9] + work * voli - vis[(size_t)7 * ne + i];
+    }
+    if (stage == 0) {
+        for (int k = 0; k < 10; k++) s.rhs[(size_t)k * ne + i] = rhs[k];
+        return;
+    }
+    // RK stage update (sr_RK.F90:29-161)
+    const double dtbyvol = s.scal[SC_DT] / m.vol[i];
+    for (int j = 0; j < 10; j++) {
+        const size_t o = (size_t)j * ne + i;
+        const double r = rhs[j] - s.d24[o];
+        const double cu = s.cu[o];
+        double cn = s.cn[o];
+        if (stage == 1) {
+            cn = cu - (11.0 / 20.0) * dtbyvol * r;
+            s.cn1[o] = cn;
+        } else if (stage == 2) {
+            cn = (3.0 / 8.0) * cu + (5.0 / 8.0) * (cn - (11.0 / 20.0) * dtbyvol * r);
+        } else if (stage == 3) {
+            cn = (4.0 / 9.0) * cu + (5.0 / 9.0) * (cn - (11.0 / 20.0) * dtbyvol * r);
+        } else {
+            cn = (111.0 / 1331.0) * cu + (260.0 / 1331.0) * s.cn1[o] +
+                 (960.0 / 1331.0) * (cn - (11.0 / 20.0) * dtbyvol * r);
+        }
+        s.cn[o] = cn;
+    }
+}
+
+// stage update from a stored residual (sr_RK.F90), for the per-routine path
+__global__ void __launch_bounds__(256) k_rk_update(State s, Mesh m, int stage)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ne = m.ne;
+    if (i >= ne || m.iblank[i] != 1) return;
+    const double dtbyvol = s.scal[SC_DT] / m.vol[i];
+    for (int j = 0; j < 10; j++) {
+        const size_t o = (size_t)j * ne + i;
+        const double r = s.rhs[o] - s.d24[o];
+        const double cu = s.cu[o];
+        double cn = s.cn[o];
+        if (stage == 1) {
+            cn = cu - (11.0 / 20.0) * dtbyvol * r;
+            s.cn1[o] = cn;
+        } else if (stage == 2) {
+            cn = (3.0 / 8.0) * cu + (5.0 / 8.0) * (cn - (11.0 / 20.0) * dtbyvol * r);
+        } else if (stage == 3) {
+            cn = (4.0 / 9.0) * cu + (5.0 / 9.0) * (cn - (11.0 / 20.0) * dtbyvol * r);
+        } else {
+            cn = (111.0 / 1331.0) * cu + (260.0 / 1331.0) * s.cn1[o] +
+                 (960.0 / 1331.0) * (cn - (11.0 / 20.0) * dtbyvol * r);
+        }
+        s.cn[o] = cn;
+    }
+}
+
+// end of step: max|cn-cu| of variables 1..4 per block (main.F90:603-612,
+// a max is order independent) and cu = cn (:619-625)
+__global__ void __launch_bounds__(256) k_end_step(State s, int ne, double *part)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < ne) {
+        for (int k = 0; k < 10; k++) {
+            const size_t o = (size_t)k * ne + i;
+            const double c = s.cn[o];
+            if (k < 4) d[k] = fabs(c - s.cu[o]);
+            s.cu[o] = c;
+        }
+    }
+    if (part) {
+        __shared__ double red[4][4];
+        for (int k = 0; k < 4; k++) {
+            double v = d[k];
+            for (int off = 32; off > 0; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
+            if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            double r = red[threadIdx.x][0];
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = dmax(r, red[threadIdx.x][w]);
+            part[(size_t)blockIdx.x * 4 + threadIdx.x] = r;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+void launch_geometry(const Mesh &m, double xmax, double ymax, double zmax, double ymin, double zmin,
+                     hipStream_t st)
+{
+    hipLaunchKernelGGL(k_geom_cell, dim3(nblk(m.ne, 128)), dim3(128), 0, st, m);
+    if (m.ng > 0)
+        hipLaunchKernelGGL(k_geom_ghost, dim3(nblk(m.ng, 128)), dim3(128), 0, st, m, xmax, ymax, zmax, ymin, zmin);
+}
+
+void launch_eos(const Params &P, const State &s, int passes, double *dtpart, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_eos, dim3(nblk(s.ne, 256)), dim3(256), 0, st, P, s, passes, dtpart);
+}
+
+void launch_dt_final(const Params &P, const State &s, const double *dtpart, int nparts, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, st, P, s, dtpart, nparts);
+}
+
+void launch_bc(const Params &P, const State &s, const Mesh &m, int viscous, hipStream_t st)
+{
+    if (m.ng > 0) hipLaunchKernelGGL(k_bc, dim3(nblk(m.ng, 256)), dim3(256), 0, st, P, s, m, viscous);
+}
+
+void launch_smooth(const Params &P, const State &s, const Mesh &m, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_smooth1, dim3(nblk(m.ne, 256)), dim3(256), 0, st, s, m);
+    hipLaunchKernelGGL(k_smooth2, dim3(nblk(m.ne, 256)), dim3(256), 0, st, P, s, m);
+}
+
+void launch_viscous(const Params &P, const State &s, const Mesh &m, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_viscous, dim3(nblk(m.ne, 128)), dim3(128), 0, st, P, s, m);
+}
+
+void launch_grad(const State &s, const Mesh &m, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_grad_limiter, dim3(nblk(m.ne, 128)), dim3(128), 0, st, s, m);
+}
+
+void launch_flux(const Params &P, const State &s, const Mesh &m, int stage, hipStream_t st)
+{
+    if (P.myflux == 1)
+        hipLaunchKernelGGL(k_flux<1>, dim3(nblk(m.ne, 128)), dim3(128), 0, st, P, s, m, stage);
+    else
+        hipLaunchKernelGGL(k_flux<2>, dim3(nblk(m.ne, 128)), dim3(128), 0, st, P, s, m, stage);
+}
+
+void launch_rk_update(const State &s, const Mesh &m, int stage, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rk_update, dim3(nblk(m.ne, 256)), dim3(256), 0, st, s, m, stage);
+}
+
+void launch_end_step(const State &s, double *part, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_end_step, dim3(nblk(s.ne, 256)), dim3(256), 0, st, s, s.ne, part);
+}
+
+}  // namespace hydra
+
+namespace hydra {
+
+// CFL bound from the current primitives, one minimum per block
+// (main.F90:525-546)
+__global__ void __launch_bounds__(256) k_dt_from_prims(Params P, State s, double *dtpart)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double loc = 1.0e8;
+    if (i < s.ne) {
+        const double *q = s.q;
+        const size_t nt = s.nt;
+        const double ag = sqrt((P.gamag * QV(Q_PG, i)) / QV(Q_ROG, i));
+        const double ap = sqrt((P.gamap * (QV(Q_PP, i) + P.pinf)) / QV(Q_ROP, i));
+        const double ug = QV(Q_UG, i), vg = QV(Q_VG, i), wg = QV(Q_WG, i);
+        const double up = QV(Q_UP, i), vp = QV(Q_VP, i), wp = QV(Q_WP, i);
+        const double qgi = sqrt(ug * ug + vg * vg + wg * wg);
+        const double qpi = sqrt(up * up + vp * vp + wp * wp);
+        loc = s.dl[i] / (dmax(ag, ap) + dmax(qgi, qpi));
+    }
+    __shared__ double red[4];
+    double v = wave_min(loc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = dmin(r, red[w]);
+        dtpart[blockIdx.x] = r;
+    }
+}
+
+void launch_dt_from_prims(const Params &P, const State &s, double *dtpart, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dt_from_prims, dim3(nblk(s.ne, 256)), dim3(256), 0, st, P, s, dtpart);
+}
+
+}  // namespace hydra

@@ -1,0 +1,77 @@
+"""ctypes binding of libhydra_hip.so (include/hydra_hip.h).
+
+Loading fails loudly when the library has not been built: there is no
+CPU fallback for the hot path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libhydra_hip.so")
+
+PARAM_D = ["pmax", "pmin", "tgmax", "tgmin", "ugmax", "ugmin", "vgmax", "vgmin", "wgmax", "wgmin",
+           "upmax", "upmin", "vpmax", "vpmin", "wpmax", "wpmin", "tpmax", "tpmin",
+           "pinf", "gamap", "gamag", "cpp", "cpg", "amolg", "prg",
+           "epslnmax", "epslnmin", "epslninf", "phipzero",
+           "cfl", "diap", "surfacetension", "as_", "ts"]
+PARAM_I = ["niter", "itersub", "istart", "myflux", "myorder", "precision"]
+PARAM_X = ["co2", "co4", "c23", "cmug", "rrg", "gravity", "prp", "foursigmabyd"]
+
+
+class HpParams(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_double) for n in PARAM_D] + [(n, ctypes.c_int) for n in PARAM_I]
+                + [(n, ctypes.c_double) for n in PARAM_X])
+
+
+def make_params(d: dict) -> HpParams:
+    p = HpParams()
+    for n in PARAM_D + PARAM_I:
+        setattr(p, n, d.get(n, 0))
+    return p  # derived constants left at zero: the library derives them
+
+
+class HydraError(RuntimeError):
+    pass
+
+
+_lib = None
+
+EXPORTS = ["hp_create", "hp_destroy", "hp_upload_state", "hp_download_state", "hp_primitive",
+           "hp_dt_min", "hp_set_dt", "hp_smooth", "hp_viscous", "hp_flux", "hp_rk_stage", "hp_rk",
+           "hp_end_step", "hp_step", "hp_synchronize", "hp_last_error", "hp_get_array",
+           "hp_set_array", "hp_last_timing"]
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HydraError(f"{LIB_PATH} is missing: build it first (python __graft_entry__.py build "
+                         "or python multiphase-code_amd/hydraphase/build.py)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    c_int, c_double, c_size = ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+    L.hp_create.argtypes = ([ctypes.POINTER(vp), c_int, c_int, c_int, dp, dp, dp, ip, c_int]
+                            + [ip] * 9 + [ctypes.POINTER(HpParams)])
+    L.hp_destroy.argtypes = [vp]
+    L.hp_upload_state.argtypes = [vp, dp, c_int, dp, c_double, c_int]
+    L.hp_download_state.argtypes = [vp, dp, c_int, dp, dp, ip]
+    for n in ["hp_primitive", "hp_smooth", "hp_viscous", "hp_flux", "hp_rk", "hp_synchronize"]:
+        getattr(L, n).argtypes = [vp]
+    L.hp_dt_min.argtypes = [vp, dp]
+    L.hp_set_dt.argtypes = [vp, c_double, c_double, c_int]
+    L.hp_rk_stage.argtypes = [vp, c_int]
+    L.hp_end_step.argtypes = [vp, dp]
+    L.hp_step.argtypes = [vp, c_int]
+    L.hp_last_error.argtypes = [vp]
+    L.hp_last_error.restype = ctypes.c_char_p
+    L.hp_get_array.argtypes = [vp, c_int, dp, c_size]
+    L.hp_set_array.argtypes = [vp, c_int, dp, c_size]
+    L.hp_last_timing.argtypes = [vp, dp, dp, ip]
+    for n in EXPORTS:
+        getattr(L, n).restype = getattr(L, n).restype if n == "hp_last_error" else c_int
+    _lib = L
+    return L

@@ -1,0 +1,40 @@
+"""Pin the oracle's 100-step field of the configuration-1 workload by hash.
+
+24^3 hexes x 6 = 82,944 jittered tets (seed 1), water sphere R=0.2 at the
+centre of an air stream at Mach 0.3 (air sound speed), CFL 0.5, 100 steps.
+Writes tests/golden/config1_100.json: SHA-256 of the conserved field (10 x
+neles float64, C order), the physical time, and per-variable norms.
+
+usage: python tests/golden/make_config1_hash.py
+"""
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "multiphase-code_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+from hydraphase import mesh as M  # noqa: E402
+import oracle as O  # noqa: E402
+
+m = M.box_mesh(24, 24, 24, jitter=0.1, seed=1)
+p = M.water_air_params(cfl=0.5)
+cu0, gp0 = M.sphere_state(m, p, [(0.0, 0.0, 0.0, 0.2)])
+o = O.Oracle(m, p)
+o.set_state(cu0, gp0, 0.0, 0)
+t0 = time.time()
+o.run(100)
+el = time.time() - t0
+cu, gp, t, tot = o.get_state()
+assert np.isfinite(cu).all()
+out = {"neles": m.neles, "steps": 100, "time": t, "total": tot,
+       "cu_sha256": hashlib.sha256(np.ascontiguousarray(cu).tobytes()).hexdigest(),
+       "gp_sha256": hashlib.sha256(np.ascontiguousarray(gp).tobytes()).hexdigest(),
+       "cu_l2": [float(v) for v in np.sqrt((cu ** 2).sum(axis=1))],
+       "oracle_seconds": el}
+json.dump(out, open(os.path.join(ROOT, "tests", "golden", "config1_100.json"), "w"), indent=1)
+print(json.dumps(out, indent=1))

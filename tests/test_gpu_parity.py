@@ -1,0 +1,225 @@
+"""HIP path (libhydra_hip.so) against the CPU oracle, through the C ABI.
+
+Every routine is compared from an identical input state; the FP64 path
+evaluates the reference's expressions in the reference's order without
+contraction, so the bar is bit-for-bit equality.  The full-step tests bound
+the relative L2 difference of the conserved variables by 1e-10 (the
+drop-in tolerance) and report whether it is exactly zero.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from hydraphase import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def solver_cls():
+    from hydraphase import Solver
+    return Solver
+
+
+def small_case(flux=2, seed=5, steps=1, hole=True, blank=True):
+    m = M.box_mesh(6, 5, 4, jitter=0.1, seed=seed, hole=(0.05, 0.0, 0.16) if hole else None)
+    if blank:
+        ib = m.iblank.copy()
+        ib[[7, 50, 123]] = 0
+        m.iblank = ib
+    p = M.water_air_params(cfl=0.5, viscous_gas=True, myflux=flux)
+    cu, gp = M.sphere_state(m, p, [(0.0, 0.1, 0.0, 0.22)])
+    rng = np.random.default_rng(seed)
+    for k in (1, 2, 3):
+        cu[k] += cu[0] * rng.uniform(-30.0, 30.0, m.neles)
+        cu[5 + k] += cu[5] * rng.uniform(-30.0, 30.0, m.neles)
+    gp[1] = rng.uniform(-20.0, 20.0, m.nghosts)
+    gp[2] = rng.uniform(-20.0, 20.0, m.nghosts)
+    o = O.Oracle(m, p)
+    o.set_state(cu, gp, 0.0, 0)
+    for _ in range(steps):
+        o.step()
+    return m, p, o
+
+
+def copy_state_to_gpu(o, s, m):
+    ne = m.neles
+    s.set("cn", o.array("cn")[:, :ne])
+    s.set("cu", o.array("cu")[:, :ne])
+    s.set("cn1", o.array("cn1")[:, :ne])
+    s.set("q", o.array("q")[:14])
+    g = o.array("grad")[:, :ne].reshape(13, 3, ne)
+    s.set("grad", np.concatenate([g[:10], g[12:13]]).reshape(33, ne))
+    s.set("phi", o.array("phi")[:ne])
+    s.set("vis", o.array("vis"))
+    s.set("d24", o.array("d24"))
+    s.set("d2q", o.array("d2q")[:, :ne])
+    s.set("anu", o.array("anu")[:, :ne])
+    s.set("rhs", o.array("rhs"))
+    _, _, t, tot = o.get_state()
+    s.set_dt(o.dt, t, tot)
+
+
+def assert_same(name, got, ref):
+    bad = ~((got == ref) | (np.isnan(got) & np.isnan(ref)))
+    if bad.any():
+        idx = np.argwhere(bad)[:5]
+        rel = np.abs(got - ref)[bad] / np.maximum(np.abs(ref[bad]), 1e-300)
+        raise AssertionError(f"{name}: {bad.sum()} of {bad.size} differ; first {idx.tolist()}; "
+                             f"max rel {rel.max():.3e}")
+
+
+@pytest.fixture(scope="module")
+def case():
+    m, p, o = small_case(steps=1)
+    o.call("primitive")
+    _, _, t, tot = o.get_state()
+    dt = o.call("dtmin") * p["cfl"]
+    o.set_dt(dt, t + dt, tot + 1)
+    o.call("smooth")
+    o.call("viscousgp")
+    o.call("derivative")
+    o.call("venkat")
+    return m, p, o
+
+
+def fresh(case):
+    m, p, o = case
+    s = solver_cls()(m, p)
+    copy_state_to_gpu(o, s, m)
+    return m, p, o, s
+
+
+def snapshot(o):
+    return {n: o.array(n).copy() for n in ["cn", "cu", "cn1", "d2q", "rhs", "d24", "anu", "q", "grad", "phi", "vis"]}
+
+
+def restore(o, snap, dt_t_tot):
+    for n, v in snap.items():
+        o.array(n)[...] = v
+    o.set_dt(*dt_t_tot)
+
+
+@pytest.mark.parametrize("routine", ["primitive", "dt", "smooth", "viscousgp", "flux", "rk_stage"])
+def test_routine_bitwise(case, routine):
+    m, p, o = case
+    snap = snapshot(o)
+    _, _, t, tot = o.get_state()
+    dtt = (o.dt, t, tot)
+    ne = m.neles
+    try:
+        m, p, o, s = fresh(case)
+        if routine == "primitive":
+            o.call("primitive"); s.primitive()
+            assert_same("cn", s.get("cn"), o.array("cn")[:, :ne])
+            assert_same("q", s.get("q")[:, :ne], o.array("q")[:14, :ne])
+        elif routine == "dt":
+            assert s.dt_min() == o.call("dtmin")
+        elif routine == "smooth":
+            o.call("smooth"); s.smooth()
+            assert_same("anu", s.get("anu"), o.array("anu")[:, :ne])
+            assert_same("d2q", s.get("d2q"), o.array("d2q")[:, :ne])
+            assert_same("d24", s.get("d24"), o.array("d24"))
+        elif routine == "viscousgp":
+            o.call("viscousgp"); s.viscousgp()
+            live = m.iblank == 1
+            assert_same("vis", s.get("vis")[:, live], o.array("vis")[:, live])
+            assert_same("ghost q", s.get("q")[:, ne:], o.array("q")[:14, ne:])
+        elif routine == "flux":
+            for flux in (2, 1):
+                restore(o, snap, dtt)
+                if flux == 1:
+                    p1 = dict(p, myflux=1)
+                    o1 = O.Oracle(m, p1)
+                    restore(o1, snap, dtt)
+                    s1 = solver_cls()(m, p1)
+                    copy_state_to_gpu(o1, s1, m)
+                    oo, ss = o1, s1
+                else:
+                    oo, ss = o, s
+                oo.call("flux_solver"); ss.flux()
+                live = m.iblank == 1
+                assert_same(f"rhs myflux={flux}", ss.get("rhs")[:, live], oo.array("rhs")[:, live])
+                assert_same(f"cn myflux={flux}", ss.get("cn"), oo.array("cn")[:, :ne])
+                assert_same(f"q myflux={flux}", ss.get("q")[:, :ne], oo.array("q")[:14, :ne])
+                assert_same(f"ghost q myflux={flux}", ss.get("q")[:, ne:], oo.array("q")[:14, ne:])
+                g = oo.array("grad")[:, :ne].reshape(13, 3, ne)
+                gref = np.concatenate([g[:10], g[12:13]]).reshape(33, ne)
+                assert_same(f"grad myflux={flux}", ss.get("grad")[:, live], gref[:, live])
+                assert_same(f"phi myflux={flux}", ss.get("phi")[live], oo.array("phi")[:ne][live])
+        elif routine == "rk_stage":
+            o.call("flux_solver"); s.flux()
+            for st in (1, 2, 3, 4):
+                o.call("rk_stage", st); s.rk_stage(st)
+                assert_same(f"cn stage {st}", s.get("cn"), o.array("cn")[:, :ne])
+            assert_same("cn1", s.get("cn1"), o.array("cn1")[:, :ne])
+    finally:
+        restore(o, snap, dtt)
+
+
+def run_pair(m, p, o, s, nsteps, fused=True):
+    """nsteps on both; returns GPU and oracle restart states."""
+    if fused:
+        s.step(nsteps)
+    else:
+        for _ in range(nsteps):
+            s.primitive()
+            dtmin = s.dt_min()
+            _, _, t, tot = s.download()
+            dt = dtmin * p["cfl"]
+            s.set_dt(dt, t + dt, tot + 1)
+            s.smooth()
+            s.rk()
+            s.end_step()
+    o.run(nsteps)
+    return s.download(), o.get_state()
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("flux", [2, 1])
+def test_steps_bitwise(flux, fused):
+    m, p, o = small_case(flux=flux, steps=0)
+    s = solver_cls()(m, p)
+    cu0, gp0, t0, tot0 = o.get_state()
+    s.upload(cu0, gp0, t0, tot0)
+    (cu, gp, t, tot), (cur, gpr, tr, totr) = run_pair(m, p, o, s, 5, fused=fused)
+    assert tot == totr == 5
+    assert t == tr
+    assert_same("cu", cu, cur)
+    assert_same("ghost prims", gp, gpr)
+
+
+def test_config1_100_steps():
+    """Configuration 1 workload: 24^3 hexes x 6 = 82,944 tets, jittered,
+    water sphere R=0.2 in an air stream at Mach 0.3, CFL 0.5, 100 steps.
+    The oracle's 100-step field is pinned by its SHA-256
+    (tests/golden/config1_100.json, written by tests/golden/make_config1_hash.py);
+    if the GPU field's hash differs, the oracle runs live and the relative
+    L2 per conserved variable must stay <= 1e-10."""
+    spec = json.load(open(os.path.join(HERE, "golden", "config1_100.json")))
+    m = M.box_mesh(24, 24, 24, jitter=0.1, seed=1)
+    p = M.water_air_params(cfl=0.5)
+    cu0, gp0 = M.sphere_state(m, p, [(0.0, 0.0, 0.0, 0.2)])
+    s = solver_cls()(m, p)
+    s.upload(cu0, gp0, 0.0, 0)
+    s.step(100)
+    cu, gp, t, tot = s.download()
+    assert tot == 100
+    h = hashlib.sha256(np.ascontiguousarray(cu).tobytes()).hexdigest()
+    print(f"config1 100 steps: gpu sha256 {h[:16]} oracle {spec['cu_sha256'][:16]} time {t!r}")
+    if h == spec["cu_sha256"]:
+        assert t == spec["time"]
+        return
+    o = O.Oracle(m, p)
+    o.set_state(cu0, gp0, 0.0, 0)
+    o.run(100)
+    cur, _, tr, _ = o.get_state()
+    rel = np.sqrt(((cu - cur) ** 2).sum(axis=1) / (cur ** 2).sum(axis=1))
+    print("relative L2 per variable:", rel)
+    assert np.all(rel <= 1e-10), rel
