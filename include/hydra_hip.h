@@ -102,6 +102,10 @@ int hp_end_step(hp_ctx *ctx, double *diffs);
  * applied on the device; no host synchronisation. */
 int hp_step(hp_ctx *ctx, int nsteps);
 
+/* The 14 primitives (order of hp_get_array id 7) of one cell, 1-based id;
+ * the driver's Velocity.dat probe |u_g(1)| (main.F90:640-642, :654). */
+int hp_probe_cell(hp_ctx *ctx, int cell, double *prims);
+
 int hp_synchronize(hp_ctx *ctx);
 const char *hp_last_error(const hp_ctx *ctx);
 

@@ -446,6 +446,16 @@ int hp_last_timing(hp_ctx *c, double *total_ms, double *flux_ms, int *flux_launc
     return HP_OK;
 }
 
+int hp_probe_cell(hp_ctx *c, int cell, double *prims)
+{
+    if (!c || !prims || cell < 1 || cell > c->s.nt) return fail(c, HP_ERR_ARG, "hp_probe_cell: bad argument");
+    for (int k = 0; k < Q_N; k++)
+        HIPCHK(c, hipMemcpyAsync(prims + k, c->s.q + (size_t)k * c->s.nt + (cell - 1), sizeof(double),
+                                 hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return HP_OK;
+}
+
 static int array_view(hp_ctx *c, int which, double **ptr, size_t *n)
 {
     const size_t ne = c->m.ne, nt = c->s.nt;

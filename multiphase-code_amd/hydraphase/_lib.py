@@ -41,7 +41,7 @@ _lib = None
 EXPORTS = ["hp_create", "hp_destroy", "hp_upload_state", "hp_download_state", "hp_primitive",
            "hp_dt_min", "hp_set_dt", "hp_smooth", "hp_viscous", "hp_flux", "hp_rk_stage", "hp_rk",
            "hp_end_step", "hp_step", "hp_synchronize", "hp_last_error", "hp_get_array",
-           "hp_set_array", "hp_last_timing"]
+           "hp_set_array", "hp_last_timing", "hp_probe_cell"]
 
 
 def lib():
@@ -71,6 +71,7 @@ def lib():
     L.hp_get_array.argtypes = [vp, c_int, dp, c_size]
     L.hp_set_array.argtypes = [vp, c_int, dp, c_size]
     L.hp_last_timing.argtypes = [vp, dp, dp, ip]
+    L.hp_probe_cell.argtypes = [vp, c_int, dp]
     for n in EXPORTS:
         getattr(L, n).restype = getattr(L, n).restype if n == "hp_last_error" else c_int
     _lib = L

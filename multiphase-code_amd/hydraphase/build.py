@@ -47,5 +47,25 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(ROOT, "fortran")
+DRIVER = os.path.join(PKG, "hydra_driver")
+
+
+def build_driver() -> str:
+    """The Fortran drop-in driver, linked against libhydra_hip.so."""
+    fc = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+    mod_dir = os.path.join(PKG, "_build", "fortran")
+    os.makedirs(mod_dir, exist_ok=True)
+    cmd = [fc, "-O2", "-cpp", f"-J{mod_dir}", os.path.join(FORTRAN, "hydra_hip_mod.f90"),
+           os.path.join(FORTRAN, "hydra_driver.F90"), f"-L{PKG}", "-lhydra_hip",
+           "-Wl,-rpath,$ORIGIN", "-o", DRIVER + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=mod_dir)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_driver())
