@@ -136,10 +136,10 @@ program hydra_driver
   ! main.F90:812-828 (17 significant digits so the values round-trip)
   call execute_command_line('mkdir -p restart_files')
   open(newunit=u, file='restart_files/restart0000.in', status='replace', action='write')
-  write(u, '(ES25.17E3)') time
-  write(u, '(*(ES25.17E3))') ((cu(i, k), i = 1, neles), k = 1, 10)
+  write(u, '(1X,ES24.16E3)') time
+  write(u, '(*(1X,ES24.16E3))') ((cu(i, k), i = 1, neles), k = 1, 10)
   do k = 1, 6
-    write(u, '(*(ES25.17E3))') (gp(i, k), i = 1, nghosts)
+    write(u, '(*(1X,ES24.16E3))') (gp(i, k), i = 1, nghosts)
   end do
   close(u)
   open(newunit=u, file='itter.in', status='replace')
