@@ -79,12 +79,24 @@ def main():
     ap.add_argument("--n", type=int, nargs=3, default=[96, 96, 90])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (torch), the known-size "
+                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every case runs "
               "on one device; the multi-GPU decomposition is a design only)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calib_copy_mb:
+        import torch
+        n = args.calib_copy_mb * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     t0 = time.time()
     m, p, cu, gp = build_case(*args.n)
     print(f"# case built: {m.neles} tets, {m.nghosts} ghosts ({time.time() - t0:.1f} s)", file=sys.stderr,

@@ -50,6 +50,17 @@ __device__ __forceinline__ double fvenkat(double delp, double delm, double epssq
 
 #define QV(s, i) q[(size_t)(s) * nt + (i)]
 
+// lanes 4c..4c+3 of a wave cooperate on cell c: one face each
+__device__ __forceinline__ double quad_get(double v, int qbase, int j) { return __shfl(v, qbase + j, 64); }
+
+__device__ __forceinline__ double quad_sum4(double v, int qbase)
+{
+    // ((t0 + t1) + t2) + t3 in lane order
+    const double t0 = quad_get(v, qbase, 0), t1 = quad_get(v, qbase, 1), t2 = quad_get(v, qbase, 2),
+                 t3 = quad_get(v, qbase, 3);
+    return t0 + t1 + t2 + t3;
+}
+
 // ---------------------------------------------------------------------------
 // geometry (sr_geometry.F90:147-289), once per mesh
 // ---------------------------------------------------------------------------
@@ -481,24 +492,26 @@ __global__ void __launch_bounds__(256) k_smooth2(Params P, State s, Mesh m)
 // plus the cell (boundary face) or plus the neighbour's 4 neighbours
 // (interior face); Newtonian stress and Fourier heat flux of both phases.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(128) k_viscous(Params P, State s, Mesh m)
+// One cell per quad, one face per lane; the quad sums the four face
+// contributions in face order, vis = (((0+V1)+V2)+V3)+V4, as the reference's
+// accumulation into zeroed arrays (sr_viscousgp.F90:41-48, :797-804).
+__global__ void __launch_bounds__(256) k_viscous(Params P, State s, Mesh m)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t >> 2, f = t & 3;
     const int ne = m.ne;
     if (i >= ne || m.iblank[i] != 1) return;
+    const int qbase = (int)(threadIdx.x & 63) & ~3;
     const double *q = s.q;
     const size_t nt = s.nt;
     const int slot[8] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP};
     double acc[8];
-    for (int v = 0; v < 8; v++) acc[v] = 0.0;
-    int nci[4];
-    for (int k = 0; k < 4; k++) nci[k] = m.nc[(size_t)k * ne + i];
-    for (int f = 0; f < 4; f++) {
+    {
         const double scx = m.sc[(size_t)(3 * f) * ne + i], scy = m.sc[(size_t)(3 * f + 1) * ne + i],
                      scz = m.sc[(size_t)(3 * f + 2) * ne + i];
         const double xf = m.fc3[(size_t)(3 * f) * ne + i], yf = m.fc3[(size_t)(3 * f + 1) * ne + i],
                      zf = m.fc3[(size_t)(3 * f + 2) * ne + i];
-        const int neigh = nci[f];
+        const int neigh = m.nc[(size_t)f * ne + i];
         double fv[8];
         for (int v = 0; v < 8; v++) fv[v] = (QV(slot[v], i) + QV(slot[v], neigh)) * 0.5;
         const double phigf = (QV(Q_PHIG, i) + QV(Q_PHIG, neigh)) * 0.5;
@@ -510,11 +523,11 @@ __global__ void __launch_bounds__(128) k_viscous(Params P, State s, Mesh m)
         const double akp = amup * P.cpp / P.prp;
         int mm[8];
         int nm;
-        mm[0] = nci[0]; mm[1] = nci[1]; mm[2] = nci[2]; mm[3] = nci[3];
-        if (neigh >= ne) {
+        for (int k = 0; k < 4; k++) mm[k] = m.nc[(size_t)k * ne + i];
+        if (neigh >= ne) {  // boundary face: the cell's 4 neighbours and the cell
             mm[4] = i;
             nm = 5;
-        } else {
+        } else {            // interior face: plus the neighbour's 4 neighbours
             for (int k = 0; k < 4; k++) mm[4 + k] = m.nc[(size_t)k * ne + neigh];
             nm = 8;
         }
@@ -535,29 +548,29 @@ __global__ void __launch_bounds__(128) k_viscous(Params P, State s, Mesh m)
             a23 = a23 + dy[k] * dz[k];
         }
         const double dnr = det3(a11, a12, a13, a12, a22, a23, a13, a23, a33);
-        double gr[8][3];
-        for (int v = 0; v < 8; v++) {
-            double dv = QV(slot[v], mm[0]) - fv[v];
-            double b1 = dv * dx[0], b2 = dv * dy[0], b3 = dv * dz[0];
-            for (int k = 1; k < nm; k++) {
-                dv = QV(slot[v], mm[k]) - fv[v];
-                b1 = b1 + dv * dx[k];
-                b2 = b2 + dv * dy[k];
-                b3 = b3 + dv * dz[k];
-            }
-            gr[v][0] = det3(b1, b2, b3, a12, a22, a23, a13, a23, a33) / dnr;
-            gr[v][1] = det3(a11, a12, a13, b1, b2, b3, a13, a23, a33) / dnr;
-            gr[v][2] = det3(a11, a12, a13, a12, a22, a23, b1, b2, b3) / dnr;
-        }
-        // gas (v = 0..3) and liquid (v = 4..7) stress + heat flux
         for (int ph = 0; ph < 2; ph++) {
             const int o = 4 * ph;
+            double gr[4][3];
+            for (int v = 0; v < 4; v++) {
+                double dv = QV(slot[o + v], mm[0]) - fv[o + v];
+                double b1 = dv * dx[0], b2 = dv * dy[0], b3 = dv * dz[0];
+                for (int k = 1; k < nm; k++) {
+                    dv = QV(slot[o + v], mm[k]) - fv[o + v];
+                    b1 = b1 + dv * dx[k];
+                    b2 = b2 + dv * dy[k];
+                    b3 = b3 + dv * dz[k];
+                }
+                gr[v][0] = det3(b1, b2, b3, a12, a22, a23, a13, a23, a33) / dnr;
+                gr[v][1] = det3(a11, a12, a13, b1, b2, b3, a13, a23, a33) / dnr;
+                gr[v][2] = det3(a11, a12, a13, a12, a22, a23, b1, b2, b3) / dnr;
+            }
+            // Newtonian stress and Fourier heat flux (:761-804)
             const double mu = ph ? amup : amug, ak = ph ? akp : akg;
             const double uf = fv[o], vf = fv[o + 1], wf = fv[o + 2];
-            const double dux = gr[o][0], duy = gr[o][1], duz = gr[o][2];
-            const double dvx = gr[o + 1][0], dvy = gr[o + 1][1], dvz = gr[o + 1][2];
-            const double dwx = gr[o + 2][0], dwy = gr[o + 2][1], dwz = gr[o + 2][2];
-            const double dtx = gr[o + 3][0], dty = gr[o + 3][1], dtz = gr[o + 3][2];
+            const double dux = gr[0][0], duy = gr[0][1], duz = gr[0][2];
+            const double dvx = gr[1][0], dvy = gr[1][1], dvz = gr[1][2];
+            const double dwx = gr[2][0], dwy = gr[2][1], dwz = gr[2][2];
+            const double dtx = gr[3][0], dty = gr[3][1], dtz = gr[3][2];
             const double delv = dux + dvy + dwz;
             const double txx = mu * (2.0 * dux - P.c23 * delv);
             const double tyy = mu * (2.0 * dvy - P.c23 * delv);
@@ -569,13 +582,17 @@ __global__ void __launch_bounds__(128) k_viscous(Params P, State s, Mesh m)
             const double qyy = uf * txy + vf * tyy + wf * tyz + ak * dty;
             const double qzz = uf * txz + vf * tyz + wf * tzz + ak * dtz;
             const double phf = ph ? phipf : phigf;
-            acc[o + 0] = acc[o + 0] + phf * (txx * scx + txy * scy + txz * scz);
-            acc[o + 1] = acc[o + 1] + phf * (txy * scx + tyy * scy + tyz * scz);
-            acc[o + 2] = acc[o + 2] + phf * (txz * scx + tyz * scy + tzz * scz);
-            acc[o + 3] = acc[o + 3] + phf * (qxx * scx + qyy * scy + qzz * scz);
+            acc[o + 0] = phf * (txx * scx + txy * scy + txz * scz);
+            acc[o + 1] = phf * (txy * scx + tyy * scy + tyz * scz);
+            acc[o + 2] = phf * (txz * scx + tyz * scy + tzz * scz);
+            acc[o + 3] = phf * (qxx * scx + qyy * scy + qzz * scz);
         }
     }
-    for (int v = 0; v < 8; v++) s.vis[(size_t)v * ne + i] = acc[v];
+    for (int v = 0; v < 8; v++) {
+        const double t0 = quad_get(acc[v], qbase, 0), t1 = quad_get(acc[v], qbase, 1),
+                     t2 = quad_get(acc[v], qbase, 2), t3 = quad_get(acc[v], qbase, 3);
+        if ((v & 3) == f) s.vis[(size_t)v * ne + i] = 0.0 + t0 + t1 + t2 + t3;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -896,23 +913,31 @@ __device__ __forceinline__ void hllc_zien(const Params &P, const FState &L, cons
 
 #define GRAD(s, d, c) s_grad[(size_t)(3 * (s) + (d)) * ne + (c)]
 
+// One cell per quad of lanes, one face per lane (sr_hllcbatten.F90:77-555):
+// each lane reconstructs, clips and solves the Riemann problem of its own
+// face; the quad then sums the four face fluxes in face order through lane
+// shuffles, so every residual is the reference's (((0+F1)+F2)+F3)+F4 without
+// atomics.  The interphase source (sr_source.F90:46-155) is formed by the
+// quad (each lane contributes its neighbour's least-squares terms) and each
+// lane finishes the conserved variables k = f, f+4, f+8.
 // stage >= 1: apply the RK stage update (sr_RK.F90) from the fused residual;
 // stage == 0: store the residual in rhs (per-routine checks).
 template <int FLUX>
-__global__ void __launch_bounds__(128) k_flux(Params P, State s, Mesh m, int stage)
+__global__ void __launch_bounds__(256) k_flux(Params P, State s, Mesh m, int stage)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t >> 2, f = t & 3;
     const int ne = m.ne;
+    // all four lanes of a quad share i, so a quad leaves together
     if (i >= ne || m.iblank[i] != 1) return;
+    const int qbase = (int)(threadIdx.x & 63) & ~3;
     const double *q = s.q;
     const size_t nt = s.nt;
     const double *s_grad = s.grad;
-    double rhs[10];
-    for (int k = 0; k < 10; k++) rhs[k] = 0.0;
     const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i];
-    const double phii = s.phi[i];
-    for (int f = 0; f < 4; f++) {
-        const int neigh = m.nc[(size_t)f * ne + i];
+    const int neigh = m.nc[(size_t)f * ne + i];
+    double F[10];
+    {
         const double scx = m.sc[(size_t)(3 * f) * ne + i], scy = m.sc[(size_t)(3 * f + 1) * ne + i],
                      scz = m.sc[(size_t)(3 * f + 2) * ne + i];
         const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
@@ -926,7 +951,7 @@ __global__ void __launch_bounds__(128) k_flux(Params P, State s, Mesh m, int sta
         if (P.myorder == 2 && neigh < ne) {
             // MUSCL reconstruction with one limiter per face pair
             // (sr_hllcbatten.F90:166-224)
-            const double phimin = dmin(phii, s.phi[neigh]);
+            const double phimin = dmin(s.phi[i], s.phi[neigh]);
             const double xf0 = xf - xi, yf0 = yf - yi, zf0 = zf - zi;
             const double xfn = xf - m.xc[neigh], yfn = yf - m.yc[neigh], zfn = zf - m.zc[neigh];
 #define REC(fld, sl)                                                                                  \
@@ -941,98 +966,102 @@ __global__ void __launch_bounds__(128) k_flux(Params P, State s, Mesh m, int sta
         clip_blend(P, L);
         clip_blend(P, R);
         if (ntype == 5) {
-            // slip wall: pressure force only (sr_hllcbatten.F90:324-336)
-            rhs[1] = rhs[1] + L.phig * L.pg * enx * area;
-            rhs[2] = rhs[2] + L.phig * L.pg * eny * area;
-            rhs[3] = rhs[3] + L.phig * L.pg * enz * area;
-            rhs[6] = rhs[6] + L.phip * L.pp * enx * area;
-            rhs[7] = rhs[7] + L.phip * L.pp * eny * area;
-            rhs[8] = rhs[8] + L.phip * L.pp * enz * area;
+            // slip wall: pressure force only (sr_hllcbatten.F90:324-336); the
+            // untouched components add +0.0, an identity on a sum that
+            // started from +0.0
+            F[0] = 0.0; F[4] = 0.0; F[5] = 0.0; F[9] = 0.0;
+            F[1] = L.phig * L.pg * enx * area;
+            F[2] = L.phig * L.pg * eny * area;
+            F[3] = L.phig * L.pg * enz * area;
+            F[6] = L.phip * L.pp * enx * area;
+            F[7] = L.phip * L.pp * eny * area;
+            F[8] = L.phip * L.pp * enz * area;
+        } else if (FLUX == 1) {
+            hllc_zien(P, L, R, enx, eny, enz, area, F);
         } else {
-            double F[10];
-            if (FLUX == 1) {
-                hllc_zien(P, L, R, enx, eny, enz, area, F);
-            } else {
-                hllc_phase<true>(P, L.phig, L.rog, L.ug, L.vg, L.wg, L.tg, L.pg, R.phig, R.rog, R.ug, R.vg,
-                                 R.wg, R.tg, R.pg, enx, eny, enz, area, F);
-                hllc_phase<false>(P, L.phip, L.rop, L.up, L.vp, L.wp, L.tp, L.pp, R.phip, R.rop, R.up, R.vp,
-                                  R.wp, R.tp, R.pp, enx, eny, enz, area, F + 5);
-            }
-            for (int k = 0; k < 10; k++) rhs[k] = rhs[k] + F[k];
+            hllc_phase<true>(P, L.phig, L.rog, L.ug, L.vg, L.wg, L.tg, L.pg, R.phig, R.rog, R.ug, R.vg, R.wg,
+                             R.tg, R.pg, enx, eny, enz, area, F);
+            hllc_phase<false>(P, L.phip, L.rop, L.up, L.vp, L.wp, L.tp, L.pp, R.phip, R.rop, R.up, R.vp,
+                              R.wp, R.tp, R.pp, enx, eny, enz, area, F + 5);
         }
+    }
+    double rhs[10];
+    for (int k = 0; k < 10; k++) {
+        const double t0 = quad_get(F[k], qbase, 0), t1 = quad_get(F[k], qbase, 1),
+                     t2 = quad_get(F[k], qbase, 2), t3 = quad_get(F[k], qbase, 3);
+        rhs[k] = 0.0 + t0 + t1 + t2 + t3;
     }
     // interphase source (sr_source.F90:46-155)
+    const double phig = QV(Q_PHIG, i), phip = QV(Q_PHIP, i), rog = QV(Q_ROG, i), rop = QV(Q_ROP, i);
+    const double ug = QV(Q_UG, i), vg = QV(Q_VG, i), wg = QV(Q_WG, i), up = QV(Q_UP, i), vp = QV(Q_VP, i),
+                 wp = QV(Q_WP, i), tg = QV(Q_TG, i), tp = QV(Q_TP, i), pg = QV(Q_PG, i);
+    const double voli = m.vol[i], dl = m.dl[i];
+    const double amug = P.as * sqrt(tg) / (1.0 + P.ts / tg);
+    const double du = ug - up, dv = vg - vp, dw = wg - wp;
+    const double drag = 0.44 * phig * (phig - 1.0) * rog * sqrt(du * du + dv * dv + dw * dw) * dl;
+    const double fdgx = drag * du, fdgy = drag * dv, fdgz = drag * dw;
+    const double qsource = 6.0 * phip * amug * P.cpg * 2.0 * (tp - tg) / (P.prg * (dl * dl));
+    const double delpstar = 2.0 * (phig * rog * phip * rop) / (phig * rop + phip * rog) * (du * du + dv * dv + dw * dw);
+    const double pint = pg - dmin(delpstar, 0.01 * pg);
+    const double den = phig * rog + (1.0 - phig) * rop;
+    const double uint = (phig * rog * ug + (1.0 - phig) * rop * up) / den;
+    const double vint = (phig * rog * vg + (1.0 - phig) * rop * vp) / den;
+    const double wint = (phig * rog * wg + (1.0 - phig) * rop * wp) / den;
+    double a1, a2, a3, b1, b2, b3, c1, c2, c3;
     {
-        const double phig = QV(Q_PHIG, i), phip = QV(Q_PHIP, i), rog = QV(Q_ROG, i), rop = QV(Q_ROP, i);
-        const double ug = QV(Q_UG, i), vg = QV(Q_VG, i), wg = QV(Q_WG, i), up = QV(Q_UP, i), vp = QV(Q_VP, i),
-                     wp = QV(Q_WP, i), tg = QV(Q_TG, i), tp = QV(Q_TP, i), pg = QV(Q_PG, i);
-        const double voli = m.vol[i], dl = m.dl[i];
-        const double amug = P.as * sqrt(tg) / (1.0 + P.ts / tg);
-        const double du = ug - up, dv = vg - vp, dw = wg - wp;
-        const double drag = 0.44 * phig * (phig - 1.0) * rog * sqrt(du * du + dv * dv + dw * dw) * dl;
-        const double fdgx = drag * du, fdgy = drag * dv, fdgz = drag * dw;
-        const double qsource = 6.0 * phip * amug * P.cpg * 2.0 * (tp - tg) / (P.prg * (dl * dl));
-        const double delpstar = 2.0 * (phig * rog * phip * rop) / (phig * rop + phip * rog) * (du * du + dv * dv + dw * dw);
-        const double pint = pg - dmin(delpstar, 0.01 * pg);
-        const double den = phig * rog + (1.0 - phig) * rop;
-        const double uint = (phig * rog * ug + (1.0 - phig) * rop * up) / den;
-        const double vint = (phig * rog * vg + (1.0 - phig) * rop * vp) / den;
-        const double wint = (phig * rog * wg + (1.0 - phig) * rop * wp) / den;
-        double dx[4], dy[4], dz[4], wt[4], dph[4];
-        for (int k = 0; k < 4; k++) {
-            const int nb = m.nc[(size_t)k * ne + i];
-            dx[k] = m.xc[nb] - xi;
-            dy[k] = m.yc[nb] - yi;
-            dz[k] = m.zc[nb] - zi;
-            dph[k] = QV(Q_PHIG, nb) - phig;
-            wt[k] = sqrt(dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k]);
-        }
-        const double a1 = dx[0] * dx[0] / wt[0] + dx[1] * dx[1] / wt[1] + dx[2] * dx[2] / wt[2] + dx[3] * dx[3] / wt[3];
-        const double a2 = dy[0] * dy[0] / wt[0] + dy[1] * dy[1] / wt[1] + dy[2] * dy[2] / wt[2] + dy[3] * dy[3] / wt[3];
-        const double a3 = dz[0] * dz[0] / wt[0] + dz[1] * dz[1] / wt[1] + dz[2] * dz[2] / wt[2] + dz[3] * dz[3] / wt[3];
-        const double b1 = dx[0] * dy[0] / wt[0] + dx[1] * dy[1] / wt[1] + dx[2] * dy[2] / wt[2] + dx[3] * dy[3] / wt[3];
-        const double b2 = dx[0] * dz[0] / wt[0] + dx[1] * dz[1] / wt[1] + dx[2] * dz[2] / wt[2] + dx[3] * dz[3] / wt[3];
-        const double b3 = dy[0] * dz[0] / wt[0] + dy[1] * dz[1] / wt[1] + dy[2] * dz[2] / wt[2] + dy[3] * dz[3] / wt[3];
-        const double c1 = dph[0] * dx[0] / wt[0] + dph[1] * dx[1] / wt[1] + dph[2] * dx[2] / wt[2] + dph[3] * dx[3] / wt[3];
-        const double c2 = dph[0] * dy[0] / wt[0] + dph[1] * dy[1] / wt[1] + dph[2] * dy[2] / wt[2] + dph[3] * dy[3] / wt[3];
-        const double c3 = dph[0] * dz[0] / wt[0] + dph[1] * dz[1] / wt[1] + dph[2] * dz[2] / wt[2] + dph[3] * dz[3] / wt[3];
-        const double dnr = det3(a1, b1, b2, b1, a2, b3, b2, b3, a3);
-        const double dphx = det3(c1, c2, c3, b1, a2, b3, b2, b3, a3) / dnr;
-        const double dphy = det3(a1, b1, b2, c1, c2, c3, b2, b3, a3) / dnr;
-        const double dphz = det3(a1, b1, b2, b1, a2, b3, c1, c2, c3) / dnr;
-        const double work = pint * (uint * dphx + vint * dphy + wint * dphz) + uint * fdgx + vint * fdgy +
-                             wint * fdgz + qsource;
+        const double dx = m.xc[neigh] - xi, dy = m.yc[neigh] - yi, dz = m.zc[neigh] - zi;
+        const double dph = QV(Q_PHIG, neigh) - phig;
+        const double wt = sqrt(dx * dx + dy * dy + dz * dz);
+        a1 = quad_sum4(dx * dx / wt, qbase);
+        a2 = quad_sum4(dy * dy / wt, qbase);
+        a3 = quad_sum4(dz * dz / wt, qbase);
+        b1 = quad_sum4(dx * dy / wt, qbase);
+        b2 = quad_sum4(dx * dz / wt, qbase);
+        b3 = quad_sum4(dy * dz / wt, qbase);
+        c1 = quad_sum4(dph * dx / wt, qbase);
+        c2 = quad_sum4(dph * dy / wt, qbase);
+        c3 = quad_sum4(dph * dz / wt, qbase);
+    }
+    const double dnr = det3(a1, b1, b2, b1, a2, b3, b2, b3, a3);
+    const double dphx = det3(c1, c2, c3, b1, a2, b3, b2, b3, a3) / dnr;
+    const double dphy = det3(a1, b1, b2, c1, c2, c3, b2, b3, a3) / dnr;
+    const double dphz = det3(a1, b1, b2, b1, a2, b3, c1, c2, c3) / dnr;
+    const double work = pint * (uint * dphx + vint * dphy + wint * dphz) + uint * fdgx + vint * fdgy +
+                        wint * fdgz + qsource;
+    const double dtbyvol = (stage > 0) ? s.scal[SC_DT] / voli : 0.0;
+    for (int k = f; k < 10; k += 4) {
+        double r = rhs[k];
         const double *vis = s.vis;
-        rhs[1] = rhs[1] - (fdgx + pint * dphx + phig * rog * P.gravity) * voli - vis[(size_t)0 * ne + i];
-        rhs[2] = rhs[2] - (fdgy + pint * dphy) * voli - vis[(size_t)1 * ne + i];
-        rhs[3] = rhs[3] - (fdgz + pint * dphz) * voli - vis[(size_t)2 * ne + i];
-        rhs[4] = rhs[4] - work * voli - vis[(size_t)3 * ne + i];
-        rhs[6] = rhs[6] + (fdgx + pint * dphx - phip * rop * P.gravity) * voli - vis[(size_t)4 * ne + i];
-        rhs[7] = rhs[7] + (fdgy + pint * dphy) * voli - vis[(size_t)5 * ne + i];
-        rhs[8] = rhs[8] + (fdgz + pint * dphz) * voli - vis[(size_t)6 * ne + i];
-        rhs[9] = rhs[9] + work * voli - vis[(size_t)7 * ne + i];
-    }
-    if (stage == 0) {
-        for (int k = 0; k < 10; k++) s.rhs[(size_t)k * ne + i] = rhs[k];
-        return;
-    }
-    // RK stage update (sr_RK.F90:29-161)
-    const double dtbyvol = s.scal[SC_DT] / m.vol[i];
-    for (int j = 0; j < 10; j++) {
-        const size_t o = (size_t)j * ne + i;
-        const double r = rhs[j] - s.d24[o];
+        switch (k) {
+        case 1: r = r - (fdgx + pint * dphx + phig * rog * P.gravity) * voli - vis[(size_t)0 * ne + i]; break;
+        case 2: r = r - (fdgy + pint * dphy) * voli - vis[(size_t)1 * ne + i]; break;
+        case 3: r = r - (fdgz + pint * dphz) * voli - vis[(size_t)2 * ne + i]; break;
+        case 4: r = r - work * voli - vis[(size_t)3 * ne + i]; break;
+        case 6: r = r + (fdgx + pint * dphx - phip * rop * P.gravity) * voli - vis[(size_t)4 * ne + i]; break;
+        case 7: r = r + (fdgy + pint * dphy) * voli - vis[(size_t)5 * ne + i]; break;
+        case 8: r = r + (fdgz + pint * dphz) * voli - vis[(size_t)6 * ne + i]; break;
+        case 9: r = r + work * voli - vis[(size_t)7 * ne + i]; break;
+        default: break;
+        }
+        const size_t o = (size_t)k * ne + i;
+        if (stage == 0) {
+            s.rhs[o] = r;
+            continue;
+        }
+        // RK stage update (sr_RK.F90:29-161)
+        const double rr = r - s.d24[o];
         const double cu = s.cu[o];
         double cn = s.cn[o];
         if (stage == 1) {
-            cn = cu - (11.0 / 20.0) * dtbyvol * r;
+            cn = cu - (11.0 / 20.0) * dtbyvol * rr;
             s.cn1[o] = cn;
         } else if (stage == 2) {
-            cn = (3.0 / 8.0) * cu + (5.0 / 8.0) * (cn - (11.0 / 20.0) * dtbyvol * r);
+            cn = (3.0 / 8.0) * cu + (5.0 / 8.0) * (cn - (11.0 / 20.0) * dtbyvol * rr);
         } else if (stage == 3) {
-            cn = (4.0 / 9.0) * cu + (5.0 / 9.0) * (cn - (11.0 / 20.0) * dtbyvol * r);
+            cn = (4.0 / 9.0) * cu + (5.0 / 9.0) * (cn - (11.0 / 20.0) * dtbyvol * rr);
         } else {
             cn = (111.0 / 1331.0) * cu + (260.0 / 1331.0) * s.cn1[o] +
-                 (960.0 / 1331.0) * (cn - (11.0 / 20.0) * dtbyvol * r);
+                 (960.0 / 1331.0) * (cn - (11.0 / 20.0) * dtbyvol * rr);
         }
         s.cn[o] = cn;
     }
@@ -1131,7 +1160,7 @@ void launch_smooth(const Params &P, const State &s, const Mesh &m, hipStream_t s
 
 void launch_viscous(const Params &P, const State &s, const Mesh &m, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_viscous, dim3(nblk(m.ne, 128)), dim3(128), 0, st, P, s, m);
+    hipLaunchKernelGGL(k_viscous, dim3(nblk(m.ne, 64)), dim3(256), 0, st, P, s, m);
 }
 
 void launch_grad(const State &s, const Mesh &m, hipStream_t st)
@@ -1141,10 +1170,11 @@ void launch_grad(const State &s, const Mesh &m, hipStream_t st)
 
 void launch_flux(const Params &P, const State &s, const Mesh &m, int stage, hipStream_t st)
 {
+    const int nb = nblk(m.ne, 64);  // 64 cells x 4 faces per block
     if (P.myflux == 1)
-        hipLaunchKernelGGL(k_flux<1>, dim3(nblk(m.ne, 128)), dim3(128), 0, st, P, s, m, stage);
+        hipLaunchKernelGGL(k_flux<1>, dim3(nb), dim3(256), 0, st, P, s, m, stage);
     else
-        hipLaunchKernelGGL(k_flux<2>, dim3(nblk(m.ne, 128)), dim3(128), 0, st, P, s, m, stage);
+        hipLaunchKernelGGL(k_flux<2>, dim3(nb), dim3(256), 0, st, P, s, m, stage);
 }
 
 void launch_rk_update(const State &s, const Mesh &m, int stage, hipStream_t st)
