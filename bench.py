@@ -15,6 +15,14 @@ The metric counts cell-stage updates (active cells x 4 stages) per second
 with the state resident in HBM.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
+       python bench.py summarize-counters FETCH.csv WRITE.csv CALIB_MB
+       python bench.py summarize-trace results.db
+
+roofline.traffic is the HBM bytes of one k_flux launch measured with
+rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (each run with
+--calib-copy-mb 1024 and summarised by summarize-counters into
+profiles/k_flux_box96x96x90_fp64_counters.md); FETCH_SIZE and WRITE_SIZE
+are scaled by what they read for a float64 b=2a kernel of known size.
 """
 from __future__ import annotations
 
@@ -71,7 +79,81 @@ def cpu_baseline(m, p, cu, gp, seconds_target=20.0):
                       f"{el:.1f} s"}
 
 
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC_FILE = os.path.join(PROFILES, "k_flux_box96x96x90_fp64_counters.md")
+
+
+def flux_traffic_bytes():
+    """HBM bytes per k_flux launch at this workload, as measured by the
+    FETCH_SIZE/WRITE_SIZE counter runs summarised in TRAFFIC_FILE (None if
+    that summary is absent)."""
+    import re
+    try:
+        txt = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch k_flux: (\d+)", txt)
+    return int(m.group(1)) if m else None
+
+
+def summarize_counters(fetch_csv, write_csv, calib_mb):
+    """Per-kernel FETCH_SIZE/WRITE_SIZE means (KiB per dispatch) from two
+    rocprofv3 --pmc runs, scaled by the same counters on the b=2a calibration
+    kernel that reads and writes calib_mb MiB each.  Prints markdown."""
+    import csv
+    import collections
+
+    def load(path):
+        agg = collections.defaultdict(list)
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in agg.items()}, {k: len(v) for k, v in agg.items()}
+
+    fe, nfe = load(fetch_csv)
+    wr, _ = load(write_csv)
+    cal = [k for k in fe if "elementwise" in k and ("Mul" in k or "Unary" in k)]
+    if not cal:
+        raise SystemExit("calibration kernel not found in " + fetch_csv)
+    kib = calib_mb * 1024.0
+    ffac = kib / fe[cal[0]]
+    wfac = kib / wr[cal[0]]
+    print(f"calibration kernel `{cal[0][:60]}...`: reads and writes {calib_mb} MiB; FETCH_SIZE "
+          f"{fe[cal[0]]:.0f} KiB, WRITE_SIZE {wr[cal[0]]:.0f} KiB -> factors {ffac:.3f} (fetch), "
+          f"{wfac:.3f} (write)\n")
+    print("| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM read MB (scaled) | HBM write MB (scaled) |")
+    print("|---|---|---|---|---|---|")
+    flux = None
+    for k in fe:
+        if "hydra" not in k:
+            continue
+        rd = fe[k] * ffac * 1024 / 1e6
+        wt = wr.get(k, 0.0) * wfac * 1024 / 1e6
+        name = k.split("(")[0].replace("void ", "")
+        print(f"| `{name}` | {nfe[k]} | {fe[k]:.0f} | {wr.get(k, 0.0):.0f} | {rd:.1f} | {wt:.1f} |")
+        if name.startswith("hydra::k_flux"):
+            flux = int(round((rd + wt) * 1e6))
+    if flux is not None:
+        print(f"\ntraffic_bytes_per_launch k_flux: {flux}")
+
+
+def summarize_trace(db_path, top=20):
+    """Per-kernel totals from a rocprofv3 --kernel-trace --stats database."""
+    import sqlite3
+    cur = sqlite3.connect(db_path).cursor()
+    print("| kernel | calls | total us | avg us | % |\n|---|---|---|---|---|")
+    for r in cur.execute(f"select * from top_kernels limit {int(top)}"):
+        name = r[0].replace("|", "/")
+        print(f"| `{name}` | {r[1]} | {r[2]:.1f} | {r[3]:.1f} | {r[4]:.2f} |")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "summarize-counters":
+        summarize_counters(sys.argv[2], sys.argv[3], int(sys.argv[4]))
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "summarize-trace":
+        summarize_trace(sys.argv[2])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -80,8 +162,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
-                    help="first run a plain device copy of this many MiB (torch), the known-size "
-                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
+                    help="first run a plain device copy (b=2a, float64) of this many MiB, the "
+                         "known-size kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every case runs "
@@ -89,12 +171,14 @@ def main():
         sys.exit(2)
 
     if args.calib_copy_mb:
+        # b = 2*a over float64: one elementwise kernel that reads and writes
+        # exactly calib_copy_mb MiB each, the yardstick for the counters
         import torch
         n = args.calib_copy_mb * (1 << 20) // 8
         a = torch.ones(n, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         for _ in range(3):
-            b.copy_(a)
+            torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
         del a, b
     t0 = time.time()
@@ -137,7 +221,8 @@ def main():
                                f"r=0.1, 8 water spheres, viscous, FP64",
                    "tets": m.neles, "active_cells": active, "ghosts": m.nghosts, "stages_per_step": 4},
         "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": flux_traffic_bytes() if list(args.n) == [96, 96, 90] else None,
                      "flux_avg_ms": flux_avg_ms, "bytes_per_cell": FLUX_BYTES_PER_CELL},
     }
     if not args.no_cpu_baseline:

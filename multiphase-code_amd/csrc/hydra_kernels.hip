@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "hydra_dev.h"
+#include "hydra_math.h"
 
 #define F1E8 ((double)1.0e-8f)
 #define F1E6 ((double)1.0e-6f)
@@ -49,6 +50,17 @@ __device__ __forceinline__ double fvenkat(double delp, double delm, double epssq
 }
 
 #define QV(s, i) q[(size_t)(s) * nt + (i)]
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2 cache.  The gather kernels renumber their blocks so that XCD x runs the
+// contiguous logical range [x*nb/8, (x+1)*nb/8): cells that are neighbours in
+// memory order then meet in one L2.  Their grids are a multiple of 8 blocks
+// (launch with nblk8); logical blocks past the last cell do nothing.
+__device__ __forceinline__ int xcd_block()
+{
+    const unsigned b = blockIdx.x, per = gridDim.x >> 3;
+    return (int)((b & 7u) * per + (b >> 3));
+}
 
 // lanes 4c..4c+3 of a wave cooperate on cell c: one face each
 __device__ __forceinline__ double quad_get(double v, int qbase, int j) { return __shfl(v, qbase + j, 64); }
@@ -424,7 +436,7 @@ __device__ __forceinline__ double switch_sum(const double *a, int i, const int *
 
 __global__ void __launch_bounds__(256) k_smooth1(State s, Mesh m)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
     const int ne = m.ne;
     if (i >= ne) return;
     int nb[4];
@@ -455,7 +467,7 @@ __global__ void __launch_bounds__(256) k_smooth1(State s, Mesh m)
 
 __global__ void __launch_bounds__(256) k_smooth2(Params P, State s, Mesh m)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
     const int ne = m.ne;
     if (i >= ne) return;
     int nb[4];
@@ -497,7 +509,7 @@ __global__ void __launch_bounds__(256) k_smooth2(Params P, State s, Mesh m)
 // accumulation into zeroed arrays (sr_viscousgp.F90:41-48, :797-804).
 __global__ void __launch_bounds__(256) k_viscous(Params P, State s, Mesh m)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
     const int i = t >> 2, f = t & 3;
     const int ne = m.ne;
     if (i >= ne || m.iblank[i] != 1) return;
@@ -602,7 +614,7 @@ __global__ void __launch_bounds__(256) k_viscous(Params P, State s, Mesh m)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(128) k_grad_limiter(State s, Mesh m)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
     const int ne = m.ne;
     if (i >= ne || m.iblank[i] != 1) return;
     const double *q = s.q;
@@ -617,13 +629,16 @@ __global__ void __launch_bounds__(128) k_grad_limiter(State s, Mesh m)
         dz[k] = m.zc[nb[k]] - zi;
         wt[k] = dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k];
     }
-    const double a1 = dx[0] * dx[0] / wt[0] + dx[1] * dx[1] / wt[1] + dx[2] * dx[2] / wt[2] + dx[3] * dx[3] / wt[3];
-    const double a2 = dy[0] * dy[0] / wt[0] + dy[1] * dy[1] / wt[1] + dy[2] * dy[2] / wt[2] + dy[3] * dy[3] / wt[3];
-    const double a3 = dz[0] * dz[0] / wt[0] + dz[1] * dz[1] / wt[1] + dz[2] * dz[2] / wt[2] + dz[3] * dz[3] / wt[3];
-    const double b1 = dx[0] * dy[0] / wt[0] + dx[1] * dy[1] / wt[1] + dx[2] * dy[2] / wt[2] + dx[3] * dy[3] / wt[3];
-    const double b2 = dx[0] * dz[0] / wt[0] + dx[1] * dz[1] / wt[1] + dx[2] * dz[2] / wt[2] + dx[3] * dz[3] / wt[3];
-    const double b3 = dz[0] * dy[0] / wt[0] + dz[1] * dy[1] / wt[1] + dz[2] * dy[2] / wt[2] + dz[3] * dy[3] / wt[3];
+    Divisor W[4];
+    for (int k = 0; k < 4; k++) W[k] = divisor(wt[k]);
+    const double a1 = divide(dx[0] * dx[0], W[0]) + divide(dx[1] * dx[1], W[1]) + divide(dx[2] * dx[2], W[2]) + divide(dx[3] * dx[3], W[3]);
+    const double a2 = divide(dy[0] * dy[0], W[0]) + divide(dy[1] * dy[1], W[1]) + divide(dy[2] * dy[2], W[2]) + divide(dy[3] * dy[3], W[3]);
+    const double a3 = divide(dz[0] * dz[0], W[0]) + divide(dz[1] * dz[1], W[1]) + divide(dz[2] * dz[2], W[2]) + divide(dz[3] * dz[3], W[3]);
+    const double b1 = divide(dx[0] * dy[0], W[0]) + divide(dx[1] * dy[1], W[1]) + divide(dx[2] * dy[2], W[2]) + divide(dx[3] * dy[3], W[3]);
+    const double b2 = divide(dx[0] * dz[0], W[0]) + divide(dx[1] * dz[1], W[1]) + divide(dx[2] * dz[2], W[2]) + divide(dx[3] * dz[3], W[3]);
+    const double b3 = divide(dz[0] * dy[0], W[0]) + divide(dz[1] * dy[1], W[1]) + divide(dz[2] * dy[2], W[2]) + divide(dz[3] * dy[3], W[3]);
     const double dnr = det3(a1, b1, b2, b1, a2, b3, b2, b3, a3);
+    const Divisor D = divisor(dnr);
     // 13 scalars; the first 11 feed the face reconstruction and are stored,
     // rog/rop (and pg/pp) feed the limiter below
     const int slot[13] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP, Q_PG, Q_PP, Q_PHIG, Q_ROG, Q_ROP};
@@ -633,16 +648,16 @@ __global__ void __launch_bounds__(128) k_grad_limiter(State s, Mesh m)
         const double qi = QV(sl, i);
         double dq[4];
         for (int k = 0; k < 4; k++) dq[k] = QV(sl, nb[k]) - qi;
-        const double c1 = dq[0] * dx[0] / wt[0] + dq[1] * dx[1] / wt[1] + dq[2] * dx[2] / wt[2] + dq[3] * dx[3] / wt[3];
+        const double c1 = divide(dq[0] * dx[0], W[0]) + divide(dq[1] * dx[1], W[1]) + divide(dq[2] * dx[2], W[2]) + divide(dq[3] * dx[3], W[3]);
         double c2;
         if (sl == Q_TP)  // sr_derivative.F90:225, last term without /wt4
-            c2 = dq[0] * dy[0] / wt[0] + dq[1] * dy[1] / wt[1] + dq[2] * dy[2] / wt[2] + dq[3] * dy[3];
+            c2 = divide(dq[0] * dy[0], W[0]) + divide(dq[1] * dy[1], W[1]) + divide(dq[2] * dy[2], W[2]) + dq[3] * dy[3];
         else
-            c2 = dq[0] * dy[0] / wt[0] + dq[1] * dy[1] / wt[1] + dq[2] * dy[2] / wt[2] + dq[3] * dy[3] / wt[3];
-        const double c3 = dq[0] * dz[0] / wt[0] + dq[1] * dz[1] / wt[1] + dq[2] * dz[2] / wt[2] + dq[3] * dz[3] / wt[3];
-        const double gx = det3(c1, c2, c3, b1, a2, b3, b2, b3, a3) / dnr;
-        const double gy = det3(a1, b1, b2, c1, c2, c3, b2, b3, a3) / dnr;
-        const double gz = det3(a1, b1, b2, b1, a2, b3, c1, c2, c3) / dnr;
+            c2 = divide(dq[0] * dy[0], W[0]) + divide(dq[1] * dy[1], W[1]) + divide(dq[2] * dy[2], W[2]) + divide(dq[3] * dy[3], W[3]);
+        const double c3 = divide(dq[0] * dz[0], W[0]) + divide(dq[1] * dz[1], W[1]) + divide(dq[2] * dz[2], W[2]) + divide(dq[3] * dz[3], W[3]);
+        const double gx = divide(det3(c1, c2, c3, b1, a2, b3, b2, b3, a3), D);
+        const double gy = divide(det3(a1, b1, b2, c1, c2, c3, b2, b3, a3), D);
+        const double gz = divide(det3(a1, b1, b2, b1, a2, b3, c1, c2, c3), D);
         if (sI < 11) {
             s.grad[(size_t)(3 * sI + 0) * ne + i] = gx;
             s.grad[(size_t)(3 * sI + 1) * ne + i] = gy;
@@ -925,7 +940,7 @@ __device__ __forceinline__ void hllc_zien(const Params &P, const FState &L, cons
 template <int FLUX>
 __global__ void __launch_bounds__(256) k_flux(Params P, State s, Mesh m, int stage)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
     const int i = t >> 2, f = t & 3;
     const int ne = m.ne;
     // all four lanes of a quad share i, so a quad leaves together
@@ -1128,6 +1143,8 @@ __global__ void __launch_bounds__(256) k_end_step(State s, int ne, double *part)
 // launch helpers
 // ---------------------------------------------------------------------------
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+// grid for a kernel indexed by xcd_block(): a multiple of 8 blocks
+static inline int nblk8(int n, int b) { return (nblk(n, b) + 7) & ~7; }
 
 void launch_geometry(const Mesh &m, double xmax, double ymax, double zmax, double ymin, double zmin,
                      hipStream_t st)
@@ -1154,23 +1171,23 @@ void launch_bc(const Params &P, const State &s, const Mesh &m, int viscous, hipS
 
 void launch_smooth(const Params &P, const State &s, const Mesh &m, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_smooth1, dim3(nblk(m.ne, 256)), dim3(256), 0, st, s, m);
-    hipLaunchKernelGGL(k_smooth2, dim3(nblk(m.ne, 256)), dim3(256), 0, st, P, s, m);
+    hipLaunchKernelGGL(k_smooth1, dim3(nblk8(m.ne, 256)), dim3(256), 0, st, s, m);
+    hipLaunchKernelGGL(k_smooth2, dim3(nblk8(m.ne, 256)), dim3(256), 0, st, P, s, m);
 }
 
 void launch_viscous(const Params &P, const State &s, const Mesh &m, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_viscous, dim3(nblk(m.ne, 64)), dim3(256), 0, st, P, s, m);
+    hipLaunchKernelGGL(k_viscous, dim3(nblk8(m.ne, 64)), dim3(256), 0, st, P, s, m);
 }
 
 void launch_grad(const State &s, const Mesh &m, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_grad_limiter, dim3(nblk(m.ne, 128)), dim3(128), 0, st, s, m);
+    hipLaunchKernelGGL(k_grad_limiter, dim3(nblk8(m.ne, 128)), dim3(128), 0, st, s, m);
 }
 
 void launch_flux(const Params &P, const State &s, const Mesh &m, int stage, hipStream_t st)
 {
-    const int nb = nblk(m.ne, 64);  // 64 cells x 4 faces per block
+    const int nb = nblk8(m.ne, 64);  // 64 cells x 4 faces per block
     if (P.myflux == 1)
         hipLaunchKernelGGL(k_flux<1>, dim3(nb), dim3(256), 0, st, P, s, m, stage);
     else
