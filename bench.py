@@ -45,13 +45,16 @@ HBM_PEAK_GBS = 8000.0
 # and write (stage 1) 2x1/4x10x8, cn write 10x8.  Neighbour values are
 # counted once, by their owning cell.
 FLUX_BYTES_PER_CELL = 112 + 264 + 8 + 96 + 96 + 24 + 16 + 16 + 64 + 80 + 80 + 60 + 40 + 80
+# FP32-bulk form: gradients 33x4 and limiter 4 instead of 33x8 and 8
+FLUX_BYTES_PER_CELL_F32 = FLUX_BYTES_PER_CELL - 132 - 4
 
 
-def build_case(nx, ny, nz, seed=1):
+def build_case(nx, ny, nz, seed=1, precision=0):
     from hydraphase import mesh as M
     hole = (0.0, 0.0, 0.1)
     m = M.box_mesh(nx, ny, nz, jitter=0.1, seed=seed, hole=hole)
     p = M.water_air_params(cfl=0.5, viscous_gas=True)
+    p["precision"] = precision
     spheres = M.sphere_train(seed, n=8, rmin=0.03, rmax=0.08, avoid=hole)
     cu, gp = M.sphere_state(m, p, spheres)
     return m, p, cu, gp
@@ -159,6 +162,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, nargs=3, default=[96, 96, 90])
+    ap.add_argument("--precision", type=int, default=0, choices=[0, 1],
+                    help="0: FP64 drop-in (the headline); 1: FP32-bulk variant")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -182,7 +187,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
     t0 = time.time()
-    m, p, cu, gp = build_case(*args.n)
+    m, p, cu, gp = build_case(*args.n, precision=args.precision)
     print(f"# case built: {m.neles} tets, {m.nghosts} ghosts ({time.time() - t0:.1f} s)", file=sys.stderr,
           flush=True)
     from hydraphase import Solver
@@ -203,7 +208,8 @@ def main():
     ms_per_step = total_ms / args.steps
     value = updates / (total_ms * 1e-3)
     flux_avg_ms = flux_ms / max(nflux, 1)
-    achieved = FLUX_BYTES_PER_CELL * active / (flux_avg_ms * 1e-3) / 1e9
+    bpc = FLUX_BYTES_PER_CELL if args.precision == 0 else FLUX_BYTES_PER_CELL_F32
+    achieved = bpc * active / (flux_avg_ms * 1e-3) / 1e9
     out = {
         "metric": "cell-stage updates/s @5M tets",
         "value": value,
@@ -215,15 +221,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == 0 else "f32-bulk/f64-state",
         "data": "synthetic",
         "config": {"workload": f"box {args.n[0]}x{args.n[1]}x{args.n[2]} hexes x6 tets, cylinder wall "
-                               f"r=0.1, 8 water spheres, viscous, FP64",
+                               f"r=0.1, 8 water spheres, viscous, "
+                               + ("FP64" if args.precision == 0 else "FP32 bulk"),
                    "tets": m.neles, "active_cells": active, "ghosts": m.nghosts, "stages_per_step": 4},
         "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": flux_traffic_bytes() if list(args.n) == [96, 96, 90] else None,
-                     "flux_avg_ms": flux_avg_ms, "bytes_per_cell": FLUX_BYTES_PER_CELL},
+                     "traffic": flux_traffic_bytes() if list(args.n) == [96, 96, 90] and args.precision == 0 else None,
+                     "flux_avg_ms": flux_avg_ms, "bytes_per_cell": bpc},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(m, p, cu, gp, args.cpu_seconds)

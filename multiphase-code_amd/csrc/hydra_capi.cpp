@@ -99,6 +99,8 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
         return fail(nullptr, HP_ERR_FLUX, "Invalid flux method! (myflux must be 1 or 2)");
     if (params->myorder != 1 && params->myorder != 2)
         return fail(nullptr, HP_ERR_ARG, "hp_create: myorder must be 1 or 2");
+    if (params->precision != 0 && params->precision != 1)
+        return fail(nullptr, HP_ERR_ARG, "hp_create: precision must be 0 (FP64) or 1 (FP32 bulk)");
 
     const int ne = neles, ng = nghosts, ntot = ne + ng;
     // host-side tables, 0-based
@@ -179,6 +181,9 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     ALLOC(s.q, (size_t)Q_N * ntot) ALLOC(s.grad, (size_t)3 * G_N * ne) ALLOC(s.phi, ne)
     ALLOC(s.vis, 8 * (size_t)ne) ALLOC(s.d24, 10 * (size_t)ne) ALLOC(s.d2q, 10 * (size_t)ne)
     ALLOC(s.anu, 2 * (size_t)ne) ALLOC(s.rhs, 10 * (size_t)ne) ALLOC(s.scal, SC_N) ALLOC(s.itot, 1)
+    if (params->precision == 1) {
+        ALLOC(s.g32, (size_t)3 * G_N * ne) ALLOC(s.phi32, ne)
+    }
     c->nparts = (ne + 255) / 256;
     ALLOC(c->dtpart, c->nparts)
     c->nendparts = (ne + 255) / 256;
@@ -341,7 +346,7 @@ static void flux_solver(hp_ctx *c, int stage)
 {
     launch_eos(c->P, c->s, 1, nullptr, c->st);  // primitiveplusbc, interior
     launch_bc(c->P, c->s, c->m, 0, c->st);      // primitiveplusbc, ghosts
-    if (c->P.myorder == 2) launch_grad(c->s, c->m, c->st);
+    if (c->P.myorder == 2) launch_grad(c->P, c->s, c->m, c->st);
     launch_flux(c->P, c->s, c->m, stage, c->st);
 }
 
@@ -415,7 +420,7 @@ int hp_step(hp_ctx *c, int nsteps)
         for (int st = 1; st <= 4; st++) {
             launch_eos(c->P, c->s, 1, nullptr, c->st);
             launch_bc(c->P, c->s, c->m, 0, c->st);
-            if (c->P.myorder == 2) launch_grad(c->s, c->m, c->st);
+            if (c->P.myorder == 2) launch_grad(c->P, c->s, c->m, c->st);
             HIPCHK(c, hipEventRecord(c->fev[ev++], c->st));
             launch_flux(c->P, c->s, c->m, st, c->st);
             HIPCHK(c, hipEventRecord(c->fev[ev++], c->st));

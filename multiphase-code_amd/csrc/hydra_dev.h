@@ -41,6 +41,7 @@ struct State {
     double *q;              // Q_N x nt
     double *grad;           // 3*G_N x ne
     double *phi;            // ne
+    float *g32, *phi32;     // FP32-bulk variant: gradients and limiter (else null)
     double *vis;            // 8 x ne
     double *d24, *d2q;      // 10 x ne
     double *anu;            // 2 x ne
@@ -57,7 +58,7 @@ void launch_dt_final(const Params &P, const State &s, const double *dtpart, int 
 void launch_bc(const Params &P, const State &s, const Mesh &m, int viscous, hipStream_t st);
 void launch_smooth(const Params &P, const State &s, const Mesh &m, hipStream_t st);
 void launch_viscous(const Params &P, const State &s, const Mesh &m, hipStream_t st);
-void launch_grad(const State &s, const Mesh &m, hipStream_t st);
+void launch_grad(const Params &P, const State &s, const Mesh &m, hipStream_t st);
 void launch_flux(const Params &P, const State &s, const Mesh &m, int stage, hipStream_t st);
 void launch_rk_update(const State &s, const Mesh &m, int stage, hipStream_t st);
 void launch_end_step(const State &s, double *part, hipStream_t st);

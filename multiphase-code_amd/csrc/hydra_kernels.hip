@@ -937,75 +937,18 @@ __device__ __forceinline__ void hllc_zien(const Params &P, const FState &L, cons
 // lane finishes the conserved variables k = f, f+4, f+8.
 // stage >= 1: apply the RK stage update (sr_RK.F90) from the fused residual;
 // stage == 0: store the residual in rhs (per-routine checks).
-template <int FLUX>
-__global__ void __launch_bounds__(256) k_flux(Params P, State s, Mesh m, int stage)
+// Interphase source (sr_source.F90:46-155) and the RK stage update
+// (sr_RK.F90:29-161) of the quad's cell i from its face-summed residual rhs
+// (FP64 in every precision).  Lane f of the quad finishes the conserved
+// variables k = f, f+4, f+8.  stage >= 1: update cn (and cn1); stage == 0:
+// store the residual in rhs (per-routine checks).
+__device__ __forceinline__ void source_and_update(const Params &P, const State &s, const Mesh &m, int i, int f,
+                                                  int qbase, int neigh, const double *rhs, int stage)
 {
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    const int i = t >> 2, f = t & 3;
     const int ne = m.ne;
-    // all four lanes of a quad share i, so a quad leaves together
-    if (i >= ne || m.iblank[i] != 1) return;
-    const int qbase = (int)(threadIdx.x & 63) & ~3;
     const double *q = s.q;
     const size_t nt = s.nt;
-    const double *s_grad = s.grad;
     const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i];
-    const int neigh = m.nc[(size_t)f * ne + i];
-    double F[10];
-    {
-        const double scx = m.sc[(size_t)(3 * f) * ne + i], scy = m.sc[(size_t)(3 * f + 1) * ne + i],
-                     scz = m.sc[(size_t)(3 * f + 2) * ne + i];
-        const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
-                     zf = m.fc0[(size_t)(3 * f + 2) * ne + i];
-        const double area = sqrt(scx * scx + scy * scy + scz * scz);
-        const double enx = scx / area, eny = scy / area, enz = scz / area;
-        const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
-        FState L, R;
-        load_fstate(q, nt, i, L);
-        load_fstate(q, nt, neigh, R);
-        if (P.myorder == 2 && neigh < ne) {
-            // MUSCL reconstruction with one limiter per face pair
-            // (sr_hllcbatten.F90:166-224)
-            const double phimin = dmin(s.phi[i], s.phi[neigh]);
-            const double xf0 = xf - xi, yf0 = yf - yi, zf0 = zf - zi;
-            const double xfn = xf - m.xc[neigh], yfn = yf - m.yc[neigh], zfn = zf - m.zc[neigh];
-#define REC(fld, sl)                                                                                  \
-    L.fld = L.fld + phimin * (xf0 * GRAD(sl, 0, i) + yf0 * GRAD(sl, 1, i) + zf0 * GRAD(sl, 2, i));     \
-    R.fld = R.fld + phimin * (xfn * GRAD(sl, 0, neigh) + yfn * GRAD(sl, 1, neigh) + zfn * GRAD(sl, 2, neigh));
-            REC(phig, G_PHIG) REC(ug, G_UG) REC(vg, G_VG) REC(wg, G_WG) REC(tg, G_TG) REC(pg, G_PG)
-            REC(up, G_UP) REC(vp, G_VP) REC(wp, G_WP) REC(tp, G_TP) REC(pp, G_PP)
-#undef REC
-        }
-        L.phip = 1.0 - L.phig;
-        R.phip = 1.0 - R.phig;
-        clip_blend(P, L);
-        clip_blend(P, R);
-        if (ntype == 5) {
-            // slip wall: pressure force only (sr_hllcbatten.F90:324-336); the
-            // untouched components add +0.0, an identity on a sum that
-            // started from +0.0
-            F[0] = 0.0; F[4] = 0.0; F[5] = 0.0; F[9] = 0.0;
-            F[1] = L.phig * L.pg * enx * area;
-            F[2] = L.phig * L.pg * eny * area;
-            F[3] = L.phig * L.pg * enz * area;
-            F[6] = L.phip * L.pp * enx * area;
-            F[7] = L.phip * L.pp * eny * area;
-            F[8] = L.phip * L.pp * enz * area;
-        } else if (FLUX == 1) {
-            hllc_zien(P, L, R, enx, eny, enz, area, F);
-        } else {
-            hllc_phase<true>(P, L.phig, L.rog, L.ug, L.vg, L.wg, L.tg, L.pg, R.phig, R.rog, R.ug, R.vg, R.wg,
-                             R.tg, R.pg, enx, eny, enz, area, F);
-            hllc_phase<false>(P, L.phip, L.rop, L.up, L.vp, L.wp, L.tp, L.pp, R.phip, R.rop, R.up, R.vp,
-                              R.wp, R.tp, R.pp, enx, eny, enz, area, F + 5);
-        }
-    }
-    double rhs[10];
-    for (int k = 0; k < 10; k++) {
-        const double t0 = quad_get(F[k], qbase, 0), t1 = quad_get(F[k], qbase, 1),
-                     t2 = quad_get(F[k], qbase, 2), t3 = quad_get(F[k], qbase, 3);
-        rhs[k] = 0.0 + t0 + t1 + t2 + t3;
-    }
     // interphase source (sr_source.F90:46-155)
     const double phig = QV(Q_PHIG, i), phip = QV(Q_PHIP, i), rog = QV(Q_ROG, i), rop = QV(Q_ROP, i);
     const double ug = QV(Q_UG, i), vg = QV(Q_VG, i), wg = QV(Q_WG, i), up = QV(Q_UP, i), vp = QV(Q_VP, i),
@@ -1080,6 +1023,78 @@ __global__ void __launch_bounds__(256) k_flux(Params P, State s, Mesh m, int sta
         }
         s.cn[o] = cn;
     }
+}
+
+template <int FLUX>
+__global__ void __launch_bounds__(256) k_flux(Params P, State s, Mesh m, int stage)
+{
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    const int i = t >> 2, f = t & 3;
+    const int ne = m.ne;
+    // all four lanes of a quad share i, so a quad leaves together
+    if (i >= ne || m.iblank[i] != 1) return;
+    const int qbase = (int)(threadIdx.x & 63) & ~3;
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    const double *s_grad = s.grad;
+    const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i];
+    const int neigh = m.nc[(size_t)f * ne + i];
+    double F[10];
+    {
+        const double scx = m.sc[(size_t)(3 * f) * ne + i], scy = m.sc[(size_t)(3 * f + 1) * ne + i],
+                     scz = m.sc[(size_t)(3 * f + 2) * ne + i];
+        const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
+                     zf = m.fc0[(size_t)(3 * f + 2) * ne + i];
+        const double area = sqrt(scx * scx + scy * scy + scz * scz);
+        const double enx = scx / area, eny = scy / area, enz = scz / area;
+        const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
+        FState L, R;
+        load_fstate(q, nt, i, L);
+        load_fstate(q, nt, neigh, R);
+        if (P.myorder == 2 && neigh < ne) {
+            // MUSCL reconstruction with one limiter per face pair
+            // (sr_hllcbatten.F90:166-224)
+            const double phimin = dmin(s.phi[i], s.phi[neigh]);
+            const double xf0 = xf - xi, yf0 = yf - yi, zf0 = zf - zi;
+            const double xfn = xf - m.xc[neigh], yfn = yf - m.yc[neigh], zfn = zf - m.zc[neigh];
+#define REC(fld, sl)                                                                                  \
+    L.fld = L.fld + phimin * (xf0 * GRAD(sl, 0, i) + yf0 * GRAD(sl, 1, i) + zf0 * GRAD(sl, 2, i));     \
+    R.fld = R.fld + phimin * (xfn * GRAD(sl, 0, neigh) + yfn * GRAD(sl, 1, neigh) + zfn * GRAD(sl, 2, neigh));
+            REC(phig, G_PHIG) REC(ug, G_UG) REC(vg, G_VG) REC(wg, G_WG) REC(tg, G_TG) REC(pg, G_PG)
+            REC(up, G_UP) REC(vp, G_VP) REC(wp, G_WP) REC(tp, G_TP) REC(pp, G_PP)
+#undef REC
+        }
+        L.phip = 1.0 - L.phig;
+        R.phip = 1.0 - R.phig;
+        clip_blend(P, L);
+        clip_blend(P, R);
+        if (ntype == 5) {
+            // slip wall: pressure force only (sr_hllcbatten.F90:324-336); the
+            // untouched components add +0.0, an identity on a sum that
+            // started from +0.0
+            F[0] = 0.0; F[4] = 0.0; F[5] = 0.0; F[9] = 0.0;
+            F[1] = L.phig * L.pg * enx * area;
+            F[2] = L.phig * L.pg * eny * area;
+            F[3] = L.phig * L.pg * enz * area;
+            F[6] = L.phip * L.pp * enx * area;
+            F[7] = L.phip * L.pp * eny * area;
+            F[8] = L.phip * L.pp * enz * area;
+        } else if (FLUX == 1) {
+            hllc_zien(P, L, R, enx, eny, enz, area, F);
+        } else {
+            hllc_phase<true>(P, L.phig, L.rog, L.ug, L.vg, L.wg, L.tg, L.pg, R.phig, R.rog, R.ug, R.vg, R.wg,
+                             R.tg, R.pg, enx, eny, enz, area, F);
+            hllc_phase<false>(P, L.phip, L.rop, L.up, L.vp, L.wp, L.tp, L.pp, R.phip, R.rop, R.up, R.vp,
+                              R.wp, R.tp, R.pp, enx, eny, enz, area, F + 5);
+        }
+    }
+    double rhs[10];
+    for (int k = 0; k < 10; k++) {
+        const double t0 = quad_get(F[k], qbase, 0), t1 = quad_get(F[k], qbase, 1),
+                     t2 = quad_get(F[k], qbase, 2), t3 = quad_get(F[k], qbase, 3);
+        rhs[k] = 0.0 + t0 + t1 + t2 + t3;
+    }
+    source_and_update(P, s, m, i, f, qbase, neigh, rhs, stage);
 }
 
 // stage update from a stored residual (sr_RK.F90), for the per-routine path
@@ -1175,23 +1190,40 @@ void launch_smooth(const Params &P, const State &s, const Mesh &m, hipStream_t s
     hipLaunchKernelGGL(k_smooth2, dim3(nblk8(m.ne, 256)), dim3(256), 0, st, P, s, m);
 }
 
+// the FP32-bulk kernels are defined at the end of this file
+__global__ void __launch_bounds__(256) k_viscous_f32(Params P, State s, Mesh m);
+__global__ void __launch_bounds__(128) k_grad_limiter_f32(State s, Mesh m);
+template <int FLUX> __global__ void __launch_bounds__(256) k_flux_f32(Params P, State s, Mesh m, int stage);
+
 void launch_viscous(const Params &P, const State &s, const Mesh &m, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_viscous, dim3(nblk8(m.ne, 64)), dim3(256), 0, st, P, s, m);
+    if (P.precision == 1)
+        hipLaunchKernelGGL(k_viscous_f32, dim3(nblk8(m.ne, 64)), dim3(256), 0, st, P, s, m);
+    else
+        hipLaunchKernelGGL(k_viscous, dim3(nblk8(m.ne, 64)), dim3(256), 0, st, P, s, m);
 }
 
-void launch_grad(const State &s, const Mesh &m, hipStream_t st)
+void launch_grad(const Params &P, const State &s, const Mesh &m, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_grad_limiter, dim3(nblk8(m.ne, 128)), dim3(128), 0, st, s, m);
+    if (P.precision == 1)
+        hipLaunchKernelGGL(k_grad_limiter_f32, dim3(nblk8(m.ne, 128)), dim3(128), 0, st, s, m);
+    else
+        hipLaunchKernelGGL(k_grad_limiter, dim3(nblk8(m.ne, 128)), dim3(128), 0, st, s, m);
 }
 
 void launch_flux(const Params &P, const State &s, const Mesh &m, int stage, hipStream_t st)
 {
     const int nb = nblk8(m.ne, 64);  // 64 cells x 4 faces per block
-    if (P.myflux == 1)
+    if (P.precision == 1) {
+        if (P.myflux == 1)
+            hipLaunchKernelGGL(k_flux_f32<1>, dim3(nb), dim3(256), 0, st, P, s, m, stage);
+        else
+            hipLaunchKernelGGL(k_flux_f32<2>, dim3(nb), dim3(256), 0, st, P, s, m, stage);
+    } else if (P.myflux == 1) {
         hipLaunchKernelGGL(k_flux<1>, dim3(nb), dim3(256), 0, st, P, s, m, stage);
-    else
+    } else {
         hipLaunchKernelGGL(k_flux<2>, dim3(nb), dim3(256), 0, st, P, s, m, stage);
+    }
 }
 
 void launch_rk_update(const State &s, const Mesh &m, int stage, hipStream_t st)
@@ -1239,6 +1271,485 @@ __global__ void __launch_bounds__(256) k_dt_from_prims(Params P, State s, double
 void launch_dt_from_prims(const Params &P, const State &s, double *dtpart, hipStream_t st)
 {
     hipLaunchKernelGGL(k_dt_from_prims, dim3(nblk(s.ne, 256)), dim3(256), 0, st, P, s, dtpart);
+}
+
+}  // namespace hydra
+
+// ===========================================================================
+// FP32-bulk variant (hp_params.precision = 1).  Gradients, limiter, face
+// reconstruction and the HLLC and viscous face fluxes run in FP32; every
+// difference of two FP64 fields (neighbour minus cell, face minus centroid,
+// neighbour maximum minus cell) is formed in FP64 before it is rounded, so
+// FP32 sees small well-scaled numbers.  Face fluxes are widened to FP64
+// before the residual sum; the EOS inversion, boundary conditions, dt
+// reduction, smoothing, source and RK update are the FP64 kernels above.
+// Multiply-adds may contract here and divisions use the hardware
+// reciprocal: these kernels are measured against the FP64 path, not
+// required to match it bit for bit.
+// ===========================================================================
+
+namespace hydra {
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqr(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
+
+__device__ __forceinline__ float det3f(float a11, float a21, float a31, float a12, float a22, float a32,
+                                       float a13, float a23, float a33)
+{
+#pragma clang fp contract(fast)
+    return a11 * (a22 * a33 - a23 * a32) - a21 * (a12 * a33 - a13 * a32) + a31 * (a12 * a23 - a13 * a22);
+}
+
+__device__ __forceinline__ float fvenkatf(float delp, float delm, float epssq)
+{
+#pragma clang fp contract(fast)
+    const float num = (delp * delp + epssq) * delm + 2.0f * delm * delm * delp;
+    const float den = delp * delp + 2.0f * delm * delm + delp * delm + epssq;
+    return num * frcp(den * delm);
+}
+
+// gradients (sr_derivative.F90:45-322, incl. the cTp2 defect) and limiter
+// (sr_VK.F90:32-274) in FP32, one thread per cell
+__global__ void __launch_bounds__(128) k_grad_limiter_f32(State s, Mesh m)
+{
+#pragma clang fp contract(fast)
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
+    const int ne = m.ne;
+    if (i >= ne || m.iblank[i] != 1) return;
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    int nb[4];
+    for (int k = 0; k < 4; k++) nb[k] = m.nc[(size_t)k * ne + i];
+    const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i];
+    float dx[4], dy[4], dz[4], iw[4];
+    for (int k = 0; k < 4; k++) {
+        dx[k] = (float)(m.xc[nb[k]] - xi);
+        dy[k] = (float)(m.yc[nb[k]] - yi);
+        dz[k] = (float)(m.zc[nb[k]] - zi);
+        iw[k] = frcp(dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k]);
+    }
+    float a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        a1 += dx[k] * dx[k] * iw[k];
+        a2 += dy[k] * dy[k] * iw[k];
+        a3 += dz[k] * dz[k] * iw[k];
+        b1 += dx[k] * dy[k] * iw[k];
+        b2 += dx[k] * dz[k] * iw[k];
+        b3 += dz[k] * dy[k] * iw[k];
+    }
+    const float idn = frcp(det3f(a1, b1, b2, b1, a2, b3, b2, b3, a3));
+    const int slot[13] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP, Q_PG, Q_PP, Q_PHIG, Q_ROG, Q_ROP};
+    float glim[4][3];
+    for (int sI = 0; sI < 13; sI++) {
+        const int sl = slot[sI];
+        const double qi = QV(sl, i);
+        float c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
+        for (int k = 0; k < 4; k++) {
+            const float dq = (float)(QV(sl, nb[k]) - qi);
+            c1 += dq * dx[k] * iw[k];
+            c2 += (sl == Q_TP && k == 3) ? dq * dy[k] : dq * dy[k] * iw[k];  // sr_derivative.F90:225
+            c3 += dq * dz[k] * iw[k];
+        }
+        const float gx = det3f(c1, c2, c3, b1, a2, b3, b2, b3, a3) * idn;
+        const float gy = det3f(a1, b1, b2, c1, c2, c3, b2, b3, a3) * idn;
+        const float gz = det3f(a1, b1, b2, b1, a2, b3, c1, c2, c3) * idn;
+        if (sI < 11) {
+            s.g32[(size_t)(3 * sI + 0) * ne + i] = gx;
+            s.g32[(size_t)(3 * sI + 1) * ne + i] = gy;
+            s.g32[(size_t)(3 * sI + 2) * ne + i] = gz;
+        }
+        const int li = (sl == Q_PG) ? 0 : (sl == Q_PP) ? 1 : (sl == Q_ROG) ? 2 : (sl == Q_ROP) ? 3 : -1;
+        if (li >= 0) { glim[li][0] = gx; glim[li][1] = gy; glim[li][2] = gz; }
+    }
+    const float epssq = (float)m.epssq[i];
+    float r0[4][3];
+    for (int f = 0; f < 4; f++) {
+        r0[f][0] = (float)(m.fc0[(size_t)(3 * f) * ne + i] - xi);
+        r0[f][1] = (float)(m.fc0[(size_t)(3 * f + 1) * ne + i] - yi);
+        r0[f][2] = (float)(m.fc0[(size_t)(3 * f + 2) * ne + i] - zi);
+    }
+    const int lq[4] = {Q_PG, Q_PP, Q_ROG, Q_ROP};
+    float ph = 1.0f;
+    for (int v = 0; v < 4; v++) {
+        const double qi = QV(lq[v], i);
+        double vmax = qi, vmin = qi;
+        for (int k = 0; k < 4; k++) {
+            const double qn = QV(lq[v], nb[k]);
+            vmax = dmax(vmax, qn);
+            vmin = dmin(vmin, qn);
+        }
+        const float dp = (float)(vmax - qi), dm = (float)(vmin - qi);
+        for (int f = 0; f < 4; f++) {
+            const float delf = r0[f][0] * glim[v][0] + r0[f][1] * glim[v][1] + r0[f][2] * glim[v][2];
+            const float pv = delf > 0.0f ? fvenkatf(dp, delf, epssq) : delf < 0.0f ? fvenkatf(dm, delf, epssq) : 1.0f;
+            ph = (v == 0 && f == 0) ? pv : fmn(ph, pv);
+        }
+    }
+    s.phi32[i] = ph;
+}
+
+struct FState32 {
+    float rog, ug, vg, wg, tg, phig, rop, up, vp, wp, tp, pg, pp, phip;
+};
+
+__device__ __forceinline__ void load_fstate32(const double *q, size_t nt, int i, FState32 &s)
+{
+    s.rog = (float)QV(Q_ROG, i); s.ug = (float)QV(Q_UG, i); s.vg = (float)QV(Q_VG, i);
+    s.wg = (float)QV(Q_WG, i); s.tg = (float)QV(Q_TG, i); s.phig = (float)QV(Q_PHIG, i);
+    s.rop = (float)QV(Q_ROP, i); s.up = (float)QV(Q_UP, i); s.vp = (float)QV(Q_VP, i);
+    s.wp = (float)QV(Q_WP, i); s.tp = (float)QV(Q_TP, i); s.pg = (float)QV(Q_PG, i);
+    s.pp = (float)QV(Q_PP, i);
+}
+
+// sr_hllcbatten.F90:228-318 in FP32
+__device__ __forceinline__ void clip_blend32(const Params &P, FState32 &s)
+{
+#pragma clang fp contract(fast)
+    const float emin = (float)P.epslnmin, emax = (float)P.epslnmax, ide = frcp(emax - emin);
+    if (s.phig < emin) {
+        s.phig = emin; s.phip = 1.0f - s.phig;
+        s.ug = s.up; s.vg = s.vp; s.wg = s.wp; s.tg = s.tp; s.pg = s.pp;
+    }
+    if (s.phip < emin) {
+        s.phip = emin; s.phig = 1.0f - s.phip;
+        s.up = s.ug; s.vp = s.vg; s.wp = s.wg; s.tp = s.tg; s.pp = s.pg;
+    }
+    if (s.phig >= emin && s.phig <= emax) {
+        const float psig = (s.phig - emin) * ide;
+        const float fg = -psig * psig * (2.0f * psig - 3.0f);
+        s.ug = fg * s.ug + (1.0f - fg) * s.up;
+        s.vg = fg * s.vg + (1.0f - fg) * s.vp;
+        s.wg = fg * s.wg + (1.0f - fg) * s.wp;
+        s.tg = fg * s.tg + (1.0f - fg) * s.tp;
+        s.pg = fg * s.pg + (1.0f - fg) * s.pp;
+    }
+    if (s.phip >= emin && s.phip <= emax) {
+        const float psig = (s.phip - emin) * ide;
+        const float fg = -psig * psig * (2.0f * psig - 3.0f);
+        s.up = fg * s.up + (1.0f - fg) * s.ug;
+        s.vp = fg * s.vp + (1.0f - fg) * s.vg;
+        s.wp = fg * s.wp + (1.0f - fg) * s.wg;
+        s.tp = fg * s.tp + (1.0f - fg) * s.tg;
+        s.pp = fg * s.pp + (1.0f - fg) * s.pg;
+    }
+    s.rog = s.pg * frcp((float)P.rrg * s.tg);
+    s.rop = (float)P.gamap * (s.pp + (float)P.pinf) * frcp((float)((P.gamap - 1.0) * P.cpp) * s.tp);
+}
+
+// one phase of the HLLC-Batten flux (sr_hllcbatten.F90:338-529) in FP32
+template <bool GAS>
+__device__ __forceinline__ void hllc_phase32(const Params &P, float phL, float roL, float uL, float vL, float wL,
+                                             float tL, float pL, float phR, float roR, float uR, float vR, float wR,
+                                             float tR, float pR, float enx, float eny, float enz, float area,
+                                             float *F)
+{
+#pragma clang fp contract(fast)
+    const float irL = frcp(roL), irR = frcp(roR);
+    float EL, ER, aL, aR;
+    if (GAS) {
+        const float cv = (float)(P.rrg / (P.gamag - 1.0)), g = (float)P.gamag;
+        EL = cv * tL + 0.5f * (uL * uL + vL * vL + wL * wL);
+        ER = cv * tR + 0.5f * (uR * uR + vR * vR + wR * wR);
+        aL = fsqr(g * pL * irL);
+        aR = fsqr(g * pR * irR);
+    } else {
+        const float cv = (float)(P.cpp / P.gamap), g = (float)P.gamap, pinf = (float)P.pinf;
+        EL = cv * tL + pinf * irL + 0.5f * (uL * uL + vL * vL + wL * wL);
+        ER = cv * tR + pinf * irR + 0.5f * (uR * uR + vR * vR + wR * wR);
+        aL = fsqr(g * (pL + pinf) * irL);
+        aR = fsqr(g * (pR + pinf) * irR);
+    }
+    const float HL = EL + pL * irL, HR = ER + pR * irR;
+    const float qL = uL * enx + vL * eny + wL * enz;
+    const float qR = uR * enx + vR * eny + wR * enz;
+    const float SL = fmn(fmn(qL - aL, qR - aR), 1.0e-8f);
+    const float SR = fmx(fmx(qL + aL, qR + aR), 1.0e-8f);
+    const float snr = pR - pL + roL * qL * (SL - qL) - roR * qR * (SR - qR);
+    const float sdr = roL * (SL - qL) - roR * (SR - qR);
+    const float SS = snr * frcp(sdr + 1.0e-8f);
+    const float pst = fmx(pL + roL * (qL - SL) * (qL - SS), 1.0e-6f * fmn(pL, pR));
+    const float mL = phL * roL, mR = phR * roR;
+    if (SL > 0.0f) {
+        F[0] = mL * qL * area;
+        F[1] = (mL * qL * uL + phL * pL * enx) * area;
+        F[2] = (mL * qL * vL + phL * pL * eny) * area;
+        F[3] = (mL * qL * wL + phL * pL * enz) * area;
+        F[4] = mL * qL * HL * area;
+    } else if (SS > 0.0f) {
+        const float idn = frcp((SS - SL) + 1.0e-6f);
+        const float c1 = mL, c2 = mL * uL, c3 = mL * vL, c4 = mL * wL, c5 = mL * EL, d = qL - SL;
+        const float s1 = d * c1 * idn;
+        const float s2 = (d * c2 + phL * (pL - pst) * enx) * idn;
+        const float s3 = (d * c3 + phL * (pL - pst) * eny) * idn;
+        const float s4 = (d * c4 + phL * (pL - pst) * enz) * idn;
+        const float s5 = (d * c5 + phL * (pL * qL - pst * SS)) * idn;
+        F[0] = (mL * qL + SL * (s1 - c1)) * area;
+        F[1] = (mL * qL * uL + phL * pL * enx + SL * (s2 - c2)) * area;
+        F[2] = (mL * qL * vL + phL * pL * eny + SL * (s3 - c3)) * area;
+        F[3] = (mL * qL * wL + phL * pL * enz + SL * (s4 - c4)) * area;
+        F[4] = (mL * qL * HL + SL * (s5 - c5)) * area;
+    } else if (SR >= 0.0f) {
+        const float idn = frcp((SS - SR) + 1.0e-6f);
+        const float c1 = mR, c2 = mR * uR, c3 = mR * vR, c4 = mR * wR, c5 = mR * ER, d = qR - SR;
+        const float s1 = d * c1 * idn;
+        const float s2 = (d * c2 + phR * (pR - pst) * enx) * idn;
+        const float s3 = (d * c3 + phR * (pR - pst) * eny) * idn;
+        const float s4 = (d * c4 + phR * (pR - pst) * enz) * idn;
+        const float s5 = (d * c5 + phR * (pR * qR - pst * SS)) * idn;
+        F[0] = (mR * qR + SR * (s1 - c1)) * area;
+        F[1] = (mR * qR * uR + phR * pR * enx + SR * (s2 - c2)) * area;
+        F[2] = (mR * qR * vR + phR * pR * eny + SR * (s3 - c3)) * area;
+        F[3] = (mR * qR * wR + phR * pR * enz + SR * (s4 - c4)) * area;
+        F[4] = (mR * qR * HR + SR * (s5 - c5)) * area;
+    } else {
+        F[0] = mR * qR * area;
+        F[1] = (mR * qR * uR + phR * pR * enx) * area;
+        F[2] = (mR * qR * vR + phR * pR * eny) * area;
+        F[3] = (mR * qR * wR + phR * pR * enz) * area;
+        F[4] = mR * qR * HR * area;
+    }
+}
+
+// HLLC-Zien flux (sr_hllczien.F90:327-487) in FP32
+__device__ __forceinline__ void hllc_zien32(const Params &P, const FState32 &L, const FState32 &R, float enx,
+                                            float eny, float enz, float area, float *F)
+{
+#pragma clang fp contract(fast)
+    const float cvg = (float)(P.rrg / (P.gamag - 1.0)), gg = (float)P.gamag;
+    const float cvp = (float)(P.cpp / P.gamap), gp = (float)P.gamap, pinf = (float)P.pinf;
+    const float irgL = frcp(L.rog), irgR = frcp(R.rog), irpL = frcp(L.rop), irpR = frcp(R.rop);
+    const float EGL = cvg * L.tg + 0.5f * (L.ug * L.ug + L.vg * L.vg + L.wg * L.wg);
+    const float HGL = EGL + L.pg * irgL;
+    const float QGL = L.ug * enx + L.vg * eny + L.wg * enz;
+    const float AGL = fsqr(gg * L.pg * irgL);
+    const float EGR = cvg * R.tg + 0.5f * (R.ug * R.ug + R.vg * R.vg + R.wg * R.wg);
+    const float HGR = EGR + R.pg * irgR;
+    const float QGR = R.ug * enx + R.vg * eny + R.wg * enz;
+    const float AGR = fsqr(gg * R.pg * irgR);
+    const float EPL = cvp * L.tp + pinf * irpL + 0.5f * (L.up * L.up + L.vp * L.vp + L.wp * L.wp);
+    const float HPL = EPL + L.pp * irpL;
+    const float QPL = L.up * enx + L.vp * eny + L.wp * enz;
+    const float APL = fsqr(gp * (L.pp + pinf) * irpL);
+    const float EPR = cvp * R.tp + pinf * irpR + 0.5f * (R.up * R.up + R.vp * R.vp + R.wp * R.wp);
+    const float HPR = EPR + R.pp * irpR;
+    const float QPR = R.up * enx + R.vp * eny + R.wp * enz;
+    const float APR = fsqr(gp * (R.pp + pinf) * irpR);
+    const float SL = fmn(fmn(fmn(fmn(QGL - AGL, QPL - APL), QGR - AGR), QPR - APR), 1.0e-8f);
+    const float SR = fmx(fmx(fmx(fmx(QGL + AGL, QPL + APL), QGR + AGR), QPR + APR), 1.0e-8f);
+    const float ROL = L.phig * L.rog + L.phip * L.rop;
+    const float ROQL = L.phig * L.rog * QGL + L.phip * L.rop * QPL;
+    const float QL = ROQL * frcp(ROL);
+    const float PRL = L.phig * L.pg + L.phip * L.pp;
+    const float ROR = R.phig * R.rog + R.phip * R.rop;
+    const float ROQR = R.phig * R.rog * QGR + R.phip * R.rop * QPR;
+    const float QR = ROQR * frcp(ROR);
+    const float PRR = R.phig * R.pg + R.phip * R.pp;
+    const float SSTAR = (PRR - PRL + ROQL * (SL - QL) - ROQR * (SR - QR)) *
+                        frcp(ROL * (SL - QL) - ROR * (SR - QR) + 1.0e-8f);
+    const float PGSTAR = fmx(L.pg + L.rog * (QGL - SL) * (QGL - SSTAR), 1.0e-6f * fmn(L.pg, R.pg));
+    const float PPSTAR = fmx(L.pp + L.rop * (QPL - SL) * (QPL - SSTAR), 1.0e-6f * fmn(L.pp, R.pp));
+    const bool left = SL >= 0.0f || SSTAR > 0.0f;
+    const FState32 &S = left ? L : R;
+    const float Sw = left ? SL : SR;
+    const float QG = left ? QGL : QGR, QP = left ? QPL : QPR, HG = left ? HGL : HGR, HP = left ? HPL : HPR;
+    const float EG = left ? EGL : EGR, EP = left ? EPL : EPR;
+    const float mg = S.phig * S.rog, mp = S.phip * S.rop;
+    F[0] = mg * QG * area;
+    F[1] = (mg * QG * S.ug + S.phig * S.pg * enx) * area;
+    F[2] = (mg * QG * S.vg + S.phig * S.pg * eny) * area;
+    F[3] = (mg * QG * S.wg + S.phig * S.pg * enz) * area;
+    F[4] = mg * QG * HG * area;
+    F[5] = mp * QP * area;
+    F[6] = (mp * QP * S.up + S.phip * S.pp * enx) * area;
+    F[7] = (mp * QP * S.vp + S.phip * S.pp * eny) * area;
+    F[8] = (mp * QP * S.wp + S.phip * S.pp * enz) * area;
+    F[9] = mp * QP * HP * area;
+    const bool star = left ? !(SL >= 0.0f) : (SR >= 0.0f);
+    if (star) {
+        const float idn = frcp((SSTAR - Sw) + 1.0e-6f);
+        const float dg = QG - Sw, dp = QP - Sw;
+        const float cg[5] = {mg, mg * S.ug, mg * S.vg, mg * S.wg, mg * EG};
+        const float cp[5] = {mp, mp * S.up, mp * S.vp, mp * S.wp, mp * EP};
+        const float sg[5] = {dg * cg[0] * idn, (dg * cg[1] + S.phig * (S.pg - PGSTAR) * enx) * idn,
+                             (dg * cg[2] + S.phig * (S.pg - PGSTAR) * eny) * idn,
+                             (dg * cg[3] + S.phig * (S.pg - PGSTAR) * enz) * idn,
+                             (dg * cg[4] + S.phig * (S.pg * QG - PGSTAR * SSTAR)) * idn};
+        const float sp[5] = {dp * cp[0] * idn, (dp * cp[1] + S.phip * (S.pp - PPSTAR) * enx) * idn,
+                             (dp * cp[2] + S.phip * (S.pp - PPSTAR) * eny) * idn,
+                             (dp * cp[3] + S.phip * (S.pp - PPSTAR) * enz) * idn,
+                             (dp * cp[4] + S.phip * (S.pp * QP - PPSTAR * SSTAR)) * idn};
+        for (int k = 0; k < 5; k++) {
+            F[k] += Sw * (sg[k] - cg[k]) * area;
+            F[5 + k] += Sw * (sp[k] - cp[k]) * area;
+        }
+    }
+}
+
+#define G32(sl, d, c) s.g32[(size_t)(3 * (sl) + (d)) * ne + (c)]
+
+// face loop in FP32, one cell per quad of lanes as in k_flux; the quad's
+// face fluxes are widened to FP64 and summed in face order, then the FP64
+// source and stage update run unchanged.
+template <int FLUX>
+__global__ void __launch_bounds__(256) k_flux_f32(Params P, State s, Mesh m, int stage)
+{
+#pragma clang fp contract(fast)
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    const int i = t >> 2, f = t & 3;
+    const int ne = m.ne;
+    if (i >= ne || m.iblank[i] != 1) return;
+    const int qbase = (int)(threadIdx.x & 63) & ~3;
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    const int neigh = m.nc[(size_t)f * ne + i];
+    float F[10];
+    {
+        const float scx = (float)m.sc[(size_t)(3 * f) * ne + i], scy = (float)m.sc[(size_t)(3 * f + 1) * ne + i],
+                    scz = (float)m.sc[(size_t)(3 * f + 2) * ne + i];
+        const float area = fsqr(scx * scx + scy * scy + scz * scz), ia = frcp(area);
+        const float enx = scx * ia, eny = scy * ia, enz = scz * ia;
+        const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
+        FState32 L, R;
+        load_fstate32(q, nt, i, L);
+        load_fstate32(q, nt, neigh, R);
+        if (P.myorder == 2 && neigh < ne) {
+            const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
+                         zf = m.fc0[(size_t)(3 * f + 2) * ne + i];
+            const float phimin = fmn(s.phi32[i], s.phi32[neigh]);
+            const float xf0 = (float)(xf - m.xc[i]), yf0 = (float)(yf - m.yc[i]), zf0 = (float)(zf - m.zc[i]);
+            const float xfn = (float)(xf - m.xc[neigh]), yfn = (float)(yf - m.yc[neigh]),
+                        zfn = (float)(zf - m.zc[neigh]);
+#define REC32(fld, sl)                                                                          \
+    L.fld += phimin * (xf0 * G32(sl, 0, i) + yf0 * G32(sl, 1, i) + zf0 * G32(sl, 2, i));       \
+    R.fld += phimin * (xfn * G32(sl, 0, neigh) + yfn * G32(sl, 1, neigh) + zfn * G32(sl, 2, neigh));
+            REC32(phig, G_PHIG) REC32(ug, G_UG) REC32(vg, G_VG) REC32(wg, G_WG) REC32(tg, G_TG)
+            REC32(pg, G_PG) REC32(up, G_UP) REC32(vp, G_VP) REC32(wp, G_WP) REC32(tp, G_TP) REC32(pp, G_PP)
+#undef REC32
+        }
+        L.phip = 1.0f - L.phig;
+        R.phip = 1.0f - R.phig;
+        clip_blend32(P, L);
+        clip_blend32(P, R);
+        if (ntype == 5) {
+            F[0] = 0.0f; F[4] = 0.0f; F[5] = 0.0f; F[9] = 0.0f;
+            F[1] = L.phig * L.pg * enx * area;
+            F[2] = L.phig * L.pg * eny * area;
+            F[3] = L.phig * L.pg * enz * area;
+            F[6] = L.phip * L.pp * enx * area;
+            F[7] = L.phip * L.pp * eny * area;
+            F[8] = L.phip * L.pp * enz * area;
+        } else if (FLUX == 1) {
+            hllc_zien32(P, L, R, enx, eny, enz, area, F);
+        } else {
+            hllc_phase32<true>(P, L.phig, L.rog, L.ug, L.vg, L.wg, L.tg, L.pg, R.phig, R.rog, R.ug, R.vg, R.wg,
+                               R.tg, R.pg, enx, eny, enz, area, F);
+            hllc_phase32<false>(P, L.phip, L.rop, L.up, L.vp, L.wp, L.tp, L.pp, R.phip, R.rop, R.up, R.vp,
+                                R.wp, R.tp, R.pp, enx, eny, enz, area, F + 5);
+        }
+    }
+    double rhs[10];
+    for (int k = 0; k < 10; k++) {
+        const double t0 = (double)__shfl(F[k], qbase, 64), t1 = (double)__shfl(F[k], qbase + 1, 64),
+                     t2 = (double)__shfl(F[k], qbase + 2, 64), t3 = (double)__shfl(F[k], qbase + 3, 64);
+        rhs[k] = 0.0 + t0 + t1 + t2 + t3;
+    }
+    source_and_update(P, s, m, i, f, qbase, neigh, rhs, stage);
+}
+
+// viscous face flux (sr_viscousgp.F90:450-1139) in FP32, one face per lane
+__global__ void __launch_bounds__(256) k_viscous_f32(Params P, State s, Mesh m)
+{
+#pragma clang fp contract(fast)
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    const int i = t >> 2, f = t & 3;
+    const int ne = m.ne;
+    if (i >= ne || m.iblank[i] != 1) return;
+    const int qbase = (int)(threadIdx.x & 63) & ~3;
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    const int slot[8] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP};
+    float acc[8];
+    {
+        const float scx = (float)m.sc[(size_t)(3 * f) * ne + i], scy = (float)m.sc[(size_t)(3 * f + 1) * ne + i],
+                    scz = (float)m.sc[(size_t)(3 * f + 2) * ne + i];
+        const double xf = m.fc3[(size_t)(3 * f) * ne + i], yf = m.fc3[(size_t)(3 * f + 1) * ne + i],
+                     zf = m.fc3[(size_t)(3 * f + 2) * ne + i];
+        const int neigh = m.nc[(size_t)f * ne + i];
+        double fv[8];
+        for (int v = 0; v < 8; v++) fv[v] = (QV(slot[v], i) + QV(slot[v], neigh)) * 0.5;
+        const float phigf = (float)((QV(Q_PHIG, i) + QV(Q_PHIG, neigh)) * 0.5);
+        const float phipf = (float)((QV(Q_PHIP, i) + QV(Q_PHIP, neigh)) * 0.5);
+        const float tgf = (float)fv[3];
+        const float amug = (float)P.as * fsqr(tgf) * frcp(1.0f + (float)P.ts * frcp(tgf));
+        const float akg = amug * (float)(P.cpg / P.prg);
+        const float amup = 0.001f;
+        const float akp = amup * (float)(P.cpp / P.prp);
+        const float c23 = (float)P.c23;
+        int mm[8];
+        int nm;
+        for (int k = 0; k < 4; k++) mm[k] = m.nc[(size_t)k * ne + i];
+        if (neigh >= ne) {
+            mm[4] = i;
+            nm = 5;
+        } else {
+            for (int k = 0; k < 4; k++) mm[4 + k] = m.nc[(size_t)k * ne + neigh];
+            nm = 8;
+        }
+        float dx[8], dy[8], dz[8];
+        float a11 = 0.0f, a22 = 0.0f, a33 = 0.0f, a12 = 0.0f, a13 = 0.0f, a23 = 0.0f;
+        for (int k = 0; k < nm; k++) {
+            dx[k] = (float)(m.xc[mm[k]] - xf);
+            dy[k] = (float)(m.yc[mm[k]] - yf);
+            dz[k] = (float)(m.zc[mm[k]] - zf);
+            a11 += dx[k] * dx[k];
+            a22 += dy[k] * dy[k];
+            a33 += dz[k] * dz[k];
+            a12 += dx[k] * dy[k];
+            a13 += dx[k] * dz[k];
+            a23 += dy[k] * dz[k];
+        }
+        const float idn = frcp(det3f(a11, a12, a13, a12, a22, a23, a13, a23, a33));
+        for (int ph = 0; ph < 2; ph++) {
+            const int o = 4 * ph;
+            float gr[4][3];
+            for (int v = 0; v < 4; v++) {
+                float b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
+                for (int k = 0; k < nm; k++) {
+                    const float dv = (float)(QV(slot[o + v], mm[k]) - fv[o + v]);
+                    b1 += dv * dx[k];
+                    b2 += dv * dy[k];
+                    b3 += dv * dz[k];
+                }
+                gr[v][0] = det3f(b1, b2, b3, a12, a22, a23, a13, a23, a33) * idn;
+                gr[v][1] = det3f(a11, a12, a13, b1, b2, b3, a13, a23, a33) * idn;
+                gr[v][2] = det3f(a11, a12, a13, a12, a22, a23, b1, b2, b3) * idn;
+            }
+            const float mu = ph ? amup : amug, ak = ph ? akp : akg;
+            const float uf = (float)fv[o], vf = (float)fv[o + 1], wf = (float)fv[o + 2];
+            const float delv = gr[0][0] + gr[1][1] + gr[2][2];
+            const float txx = mu * (2.0f * gr[0][0] - c23 * delv);
+            const float tyy = mu * (2.0f * gr[1][1] - c23 * delv);
+            const float tzz = mu * (2.0f * gr[2][2] - c23 * delv);
+            const float txy = mu * (gr[1][0] + gr[0][1]);
+            const float txz = mu * (gr[2][0] + gr[0][2]);
+            const float tyz = mu * (gr[1][2] + gr[2][1]);
+            const float qxx = uf * txx + vf * txy + wf * txz + ak * gr[3][0];
+            const float qyy = uf * txy + vf * tyy + wf * tyz + ak * gr[3][1];
+            const float qzz = uf * txz + vf * tyz + wf * tzz + ak * gr[3][2];
+            const float phf = ph ? phipf : phigf;
+            acc[o + 0] = phf * (txx * scx + txy * scy + txz * scz);
+            acc[o + 1] = phf * (txy * scx + tyy * scy + tyz * scz);
+            acc[o + 2] = phf * (txz * scx + tyz * scy + tzz * scz);
+            acc[o + 3] = phf * (qxx * scx + qyy * scy + qzz * scz);
+        }
+    }
+    for (int v = 0; v < 8; v++) {
+        const double t0 = (double)__shfl(acc[v], qbase, 64), t1 = (double)__shfl(acc[v], qbase + 1, 64),
+                     t2 = (double)__shfl(acc[v], qbase + 2, 64), t3 = (double)__shfl(acc[v], qbase + 3, 64);
+        if ((v & 3) == f) s.vis[(size_t)v * ne + i] = 0.0 + t0 + t1 + t2 + t3;
+    }
 }
 
 }  // namespace hydra

@@ -3,7 +3,8 @@
 24^3 hexes x 6 = 82,944 jittered tets (seed 1), water sphere R=0.2 at the
 centre of an air stream at Mach 0.3 (air sound speed), CFL 0.5, 100 steps.
 Writes tests/golden/config1_100.json: SHA-256 of the conserved field (10 x
-neles float64, C order), the physical time, and per-variable norms.
+neles float64, C order), the physical time, and per-variable norms, at step
+100 and (under "step10") at step 10.
 
 usage: python tests/golden/make_config1_hash.py
 """
@@ -27,7 +28,9 @@ cu0, gp0 = M.sphere_state(m, p, [(0.0, 0.0, 0.0, 0.2)])
 o = O.Oracle(m, p)
 o.set_state(cu0, gp0, 0.0, 0)
 t0 = time.time()
-o.run(100)
+o.run(10)
+cu10, gp10, t10, tot10 = o.get_state()
+o.run(90)
 el = time.time() - t0
 cu, gp, t, tot = o.get_state()
 assert np.isfinite(cu).all()
@@ -35,6 +38,9 @@ out = {"neles": m.neles, "steps": 100, "time": t, "total": tot,
        "cu_sha256": hashlib.sha256(np.ascontiguousarray(cu).tobytes()).hexdigest(),
        "gp_sha256": hashlib.sha256(np.ascontiguousarray(gp).tobytes()).hexdigest(),
        "cu_l2": [float(v) for v in np.sqrt((cu ** 2).sum(axis=1))],
-       "oracle_seconds": el}
+       "oracle_seconds": el,
+       "step10": {"time": t10, "total": tot10,
+                  "cu_sha256": hashlib.sha256(np.ascontiguousarray(cu10).tobytes()).hexdigest(),
+                  "cu_l2": [float(v) for v in np.sqrt((cu10 ** 2).sum(axis=1))]}}
 json.dump(out, open(os.path.join(ROOT, "tests", "golden", "config1_100.json"), "w"), indent=1)
 print(json.dumps(out, indent=1))
