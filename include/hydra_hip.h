@@ -125,6 +125,40 @@ int hp_set_array(hp_ctx *ctx, int which, const double *host, size_t count);
  * context's stream: total milliseconds and the flux kernel's share. */
 int hp_last_timing(hp_ctx *ctx, double *total_ms, double *flux_ms, int *flux_launches);
 
+/* ---------------------------------------------------------------------------
+ * Batches: many independent cases advanced by the same kernel launches.
+ * No counterpart in the reference (one case per process there); these serve
+ * ensembles of uneven cases and the request loop that admits cases by HBM
+ * budget.  A case is an hp_ctx created as above, with its own device
+ * memory; joining or leaving a batch copies no state.  All cases of a batch
+ * share precision, myflux and myorder.  Each launch walks a device table of
+ * 64-cell tiles (case slot, first cell, count), so one grid covers every
+ * case, and one captured HIP graph of a time step stays valid while cases
+ * join and leave.  Each case keeps its own dt, time and step count.
+ * While a case belongs to a batch its own hp_* calls wait for the batch's
+ * work first and complete before they return.
+ * ------------------------------------------------------------------------- */
+typedef struct hp_batch hp_batch;
+
+/* capacity: at most max_cases cases holding max_cells cells and max_ghosts
+ * boundary ghosts in total */
+int hp_batch_create(hp_batch **b, int max_cases, int max_cells, int max_ghosts);
+int hp_batch_destroy(hp_batch *b);   /* members stay valid, no longer batched */
+int hp_batch_add(hp_batch *b, hp_ctx *ctx, int *slot);
+int hp_batch_remove(hp_batch *b, hp_ctx *ctx);
+/* blocks per launch family (cell kernels, four-lanes-per-cell face kernels,
+ * ghost kernels); rounded up to multiples of 8; 0 = one block per tile at
+ * capacity.  Chosen by the tuner in hydraphase.batch. */
+int hp_batch_set_grid(hp_batch *b, int cell_blocks, int quad_blocks, int ghost_blocks);
+/* nsteps time steps of every member; use_graph: replay one captured step */
+int hp_batch_step(hp_batch *b, int nsteps, int use_graph);
+int hp_batch_synchronize(hp_batch *b);
+/* elapsed milliseconds of the last hp_batch_step (HIP events) */
+int hp_batch_last_ms(hp_batch *b, double *ms);
+const char *hp_batch_last_error(const hp_batch *b);
+/* device bytes hp_create allocates for such a case (admission by budget) */
+size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "hydra_dev.h"
@@ -17,16 +18,66 @@
 
 using namespace hydra;
 
+// Device tables that let one launch cover a set of cases (hydra_dev.h
+// Batch): per-slot copies of the Params/State/Mesh views, the cell and
+// ghost tile tables, and the per-tile partials.  A context owns one with a
+// single slot; an hp_batch owns one with many.
+struct TileSet {
+    Params *dP = nullptr;
+    State *dS = nullptr;
+    Mesh *dM = nullptr;
+    Tile *ct = nullptr, *gt = nullptr;
+    int *cnt = nullptr, *t0 = nullptr, *t1 = nullptr;
+    double *part = nullptr;
+    int slots = 0, ct_cap = 0, gt_cap = 0;
+    int nct = 0, ngt = 0;
+    std::vector<Params> hP;
+    std::vector<State> hS;
+    std::vector<Mesh> hM;
+    std::vector<char> used;
+    // host staging of the last upload (kept alive until the next one)
+    std::vector<Tile> hct, hgt;
+    std::vector<int> h0, h1, hcnt;
+
+    Batch view() const
+    {
+        Batch B{};
+        B.P = dP; B.S = dS; B.M = dM; B.ct = ct; B.gt = gt; B.cnt = cnt;
+        B.slot_t0 = t0; B.slot_t1 = t1; B.part = part;
+        if (slots == 1 && used[0]) {  // a context's own set: its case rides in the launch
+            B.P1 = hP[0];
+            B.S1 = hS[0];
+            B.M1 = hM[0];
+            B.n1 = hS[0].ne;
+            B.n1g = hM[0].ng;
+        }
+        return B;
+    }
+};
+
+struct hp_batch {
+    hipStream_t st = nullptr;
+    TileSet ts;
+    Grid G{};
+    std::vector<void *> bufs;
+    std::vector<hp_ctx *> members;  // per slot, null when free
+    int ncases = 0;
+    int grid_cell = 0, grid_quad = 0, grid_ghost = 0;  // 0: one block per tile at capacity
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::string err;
+};
+
 struct hp_ctx {
     hipStream_t st = nullptr;
     Params P{};
     Mesh m{};
     State s{};
     std::vector<void *> bufs;
-    double *dtpart = nullptr;
-    int nparts = 0;
-    double *endpart = nullptr;
-    int nendparts = 0;
+    TileSet ts;
+    Grid G{};
+    hp_batch *batch = nullptr;  // the batch this case belongs to, if any
     std::string err;
     // timing of the last hp_step
     std::vector<hipEvent_t> fev;  // start/stop pairs around flux launches
@@ -77,6 +128,91 @@ void derive_params(Params &P)
     P.gravity = 0.0;
     P.prp = 10.0;
     P.foursigmabyd = 4.0 * P.surfacetension / P.diap;
+}
+
+int tiles_of(int n) { return (n + TILE - 1) / TILE; }
+int round8(int n) { return ((n < 1 ? 1 : n) + 7) & ~7; }
+
+// device tables for `slots` slots and the given tile capacities; alloc(ptr,
+// count) is the owner's allocator (its buffers are freed with the owner)
+template <class Alloc>
+int ts_alloc(TileSet &t, int slots, int ct_cap, int gt_cap, Alloc &&alloc)
+{
+    t.slots = slots;
+    t.ct_cap = ct_cap;
+    t.gt_cap = gt_cap;
+    t.hP.assign(slots, Params{});
+    t.hS.assign(slots, State{});
+    t.hM.assign(slots, Mesh{});
+    t.used.assign(slots, 0);
+    int rc;
+    if ((rc = alloc(&t.dP, slots)) || (rc = alloc(&t.dS, slots)) || (rc = alloc(&t.dM, slots)) ||
+        (rc = alloc(&t.ct, ct_cap)) || (rc = alloc(&t.gt, gt_cap)) || (rc = alloc(&t.cnt, 2)) ||
+        (rc = alloc(&t.t0, slots)) || (rc = alloc(&t.t1, slots)) || (rc = alloc(&t.part, 4 * (size_t)ct_cap)))
+        return rc;
+    return HP_OK;
+}
+
+// rebuild the tile tables from the slots in use and copy them to the device
+int ts_upload(TileSet &t, hipStream_t st, std::string &err)
+{
+    t.hct.clear();
+    t.hgt.clear();
+    t.h0.assign(t.slots, 0);
+    t.h1.assign(t.slots, 0);
+    for (int c = 0; c < t.slots; c++) {
+        if (!t.used[c]) continue;
+        const int ne = t.hS[c].ne, ng = t.hM[c].ng;
+        t.h0[c] = (int)t.hct.size();
+        for (int f = 0; f < ne; f += TILE) t.hct.push_back(Tile{c, f, std::min(TILE, ne - f), 0});
+        t.h1[c] = (int)t.hct.size();
+        for (int f = 0; f < ng; f += TILE) t.hgt.push_back(Tile{c, f, std::min(TILE, ng - f), 0});
+    }
+    if ((int)t.hct.size() > t.ct_cap || (int)t.hgt.size() > t.gt_cap) {
+        err = "tile tables over capacity";
+        return HP_ERR_ARG;
+    }
+    t.nct = (int)t.hct.size();
+    t.ngt = (int)t.hgt.size();
+    t.hcnt = {t.nct, t.ngt};
+    auto cp = [&](void *d, const void *h, size_t bytes) -> hipError_t {
+        return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st) : hipSuccess;
+    };
+    hipError_t e;
+    if ((e = cp(t.dP, t.hP.data(), sizeof(Params) * t.slots)) != hipSuccess ||
+        (e = cp(t.dS, t.hS.data(), sizeof(State) * t.slots)) != hipSuccess ||
+        (e = cp(t.dM, t.hM.data(), sizeof(Mesh) * t.slots)) != hipSuccess ||
+        (e = cp(t.ct, t.hct.data(), sizeof(Tile) * t.hct.size())) != hipSuccess ||
+        (e = cp(t.gt, t.hgt.data(), sizeof(Tile) * t.hgt.size())) != hipSuccess ||
+        (e = cp(t.cnt, t.hcnt.data(), sizeof(int) * 2)) != hipSuccess ||
+        (e = cp(t.t0, t.h0.data(), sizeof(int) * t.slots)) != hipSuccess ||
+        (e = cp(t.t1, t.h1.data(), sizeof(int) * t.slots)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess) {
+        err = std::string("tile tables: ") + hipGetErrorString(e);
+        return HP_ERR_HIP;
+    }
+    return HP_OK;
+}
+
+// one time step (main.F90:515-625) of every case in B; fev (8 events, may
+// be null) brackets the four flux launches
+void enqueue_step(const Batch &B, const Grid &G, hipStream_t st, hipEvent_t *fev)
+{
+    launch_eos(B, G, 1, st);       // primitive + CFL bound per tile
+    launch_dt_final(B, G, st);     // dt, time, total per case
+    launch_smooth(B, G, st);       // :563
+    launch_bc(B, G, 1, st);        // sr_RK.F90:19, viscousgp's BCs
+    launch_viscous(B, G, st);
+    for (int k = 1; k <= 4; k++) {
+        launch_eos(B, G, 0, st);   // primitiveplusbc
+        launch_bc(B, G, 0, st);
+        launch_grad(B, G, st);
+        if (fev) (void)hipEventRecord(fev[2 * (k - 1)], st);
+        launch_flux(B, G, k, st);  // face flux + source + stage k update
+        if (fev) (void)hipEventRecord(fev[2 * (k - 1) + 1], st);
+    }
+    launch_eos(B, G, 0, st);       // main.F90:595
+    launch_end_step(B, G, 0, st);  // :619-625
 }
 
 }  // namespace
@@ -184,10 +320,6 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     if (params->precision == 1) {
         ALLOC(s.g32, (size_t)3 * G_N * ne) ALLOC(s.phi32, ne)
     }
-    c->nparts = (ne + 255) / 256;
-    ALLOC(c->dtpart, c->nparts)
-    c->nendparts = (ne + 255) / 256;
-    ALLOC(c->endpart, 4 * (size_t)c->nendparts)
 #undef ALLOC
     m.x = dx; m.y = dy; m.z = dz; m.nod = dnod; m.nc = dnc; m.ncp = dncp; m.iblank = dib;
     m.gpar = dgpar; m.gtype = dgtype; m.gside = dgside; m.gdd_win = dgwin;
@@ -216,6 +348,31 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
         hp_destroy(c);
         return HP_ERR_HIP;
     }
+    // the context is a batch of one: slot 0, one block per tile
+    auto alloc = [c](auto **p, size_t n) { return dalloc(c, p, n); };
+    if ((rc = ts_alloc(c->ts, 1, tiles_of(ne), std::max(1, tiles_of(ng)), alloc)) != HP_OK) {
+        g_create_err = c->err;
+        hp_destroy(c);
+        return rc;
+    }
+    c->ts.hP[0] = c->P;
+    c->ts.hS[0] = s;
+    c->ts.hM[0] = m;
+    c->ts.used[0] = 1;
+    if ((rc = ts_upload(c->ts, c->st, c->err)) != HP_OK) {
+        g_create_err = c->err;
+        hp_destroy(c);
+        return rc;
+    }
+    c->G.cell = round8(c->ts.nct);
+    c->G.quad = round8(c->ts.nct);
+    c->G.ghost = round8(c->ts.ngt);
+    c->G.slots = 1;
+    c->G.persist = 0;
+    c->G.single = 1;
+    c->G.precision = c->P.precision;
+    c->G.myflux = c->P.myflux;
+    c->G.myorder = c->P.myorder;
     *out = c;
     return HP_OK;
 }
@@ -223,6 +380,7 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
 int hp_destroy(hp_ctx *c)
 {
     if (!c) return HP_OK;
+    if (c->batch) (void)hp_batch_remove(c->batch, c);
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (void *p : c->bufs) (void)hipFree(p);
     for (hipEvent_t e : c->fev) (void)hipEventDestroy(e);
@@ -296,23 +454,35 @@ static int check_launch(hp_ctx *c, const char *what)
     return HP_OK;
 }
 
+// A case that belongs to a batch is also advanced on the batch's stream:
+// its own routines first wait for the batch and finish before returning.
+static int sync_owner(hp_ctx *c);
+static int finish(hp_ctx *c, const char *what)
+{
+    if (int rc = check_launch(c, what)) return rc;
+    if (c->batch) HIPCHK(c, hipStreamSynchronize(c->st));
+    return HP_OK;
+}
+
 int hp_primitive(hp_ctx *c)
 {
     if (!c) return fail(nullptr, HP_ERR_ARG, "null context");
-    launch_eos(c->P, c->s, 1, nullptr, c->st);
-    return check_launch(c, "primitive");
+    if (int rc = sync_owner(c)) return rc;
+    launch_eos(c->ts.view(), c->G, 0, c->st);
+    return finish(c, "primitive");
 }
 
 int hp_dt_min(hp_ctx *c, double *dtmin)
 {
     if (!c || !dtmin) return fail(c, HP_ERR_ARG, "hp_dt_min: bad argument");
-    launch_dt_from_prims(c->P, c->s, c->dtpart, c->st);
+    if (int rc = sync_owner(c)) return rc;
+    launch_dt_from_prims(c->ts.view(), c->G, c->st);
     if (int rc = check_launch(c, "dt")) return rc;
-    std::vector<double> part(c->nparts);
-    HIPCHK(c, hipMemcpyAsync(part.data(), c->dtpart, sizeof(double) * c->nparts, hipMemcpyDeviceToHost, c->st));
+    std::vector<double> part(4 * (size_t)c->ts.nct);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->ts.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     double r = 1.0e8;
-    for (double v : part) r = v < r ? v : r;
+    for (int t = 0; t < c->ts.nct; t++) r = part[(size_t)4 * t] < r ? part[(size_t)4 * t] : r;
     *dtmin = r;
     return HP_OK;
 }
@@ -330,64 +500,71 @@ int hp_set_dt(hp_ctx *c, double dt, double time, int total)
 int hp_smooth(hp_ctx *c)
 {
     if (!c) return fail(nullptr, HP_ERR_ARG, "null context");
-    launch_smooth(c->P, c->s, c->m, c->st);
-    return check_launch(c, "smooth");
+    if (int rc = sync_owner(c)) return rc;
+    launch_smooth(c->ts.view(), c->G, c->st);
+    return finish(c, "smooth");
 }
 
 int hp_viscous(hp_ctx *c)
 {
     if (!c) return fail(nullptr, HP_ERR_ARG, "null context");
-    launch_bc(c->P, c->s, c->m, 1, c->st);
-    launch_viscous(c->P, c->s, c->m, c->st);
-    return check_launch(c, "viscous");
+    if (int rc = sync_owner(c)) return rc;
+    launch_bc(c->ts.view(), c->G, 1, c->st);
+    launch_viscous(c->ts.view(), c->G, c->st);
+    return finish(c, "viscous");
 }
 
 static void flux_solver(hp_ctx *c, int stage)
 {
-    launch_eos(c->P, c->s, 1, nullptr, c->st);  // primitiveplusbc, interior
-    launch_bc(c->P, c->s, c->m, 0, c->st);      // primitiveplusbc, ghosts
-    if (c->P.myorder == 2) launch_grad(c->P, c->s, c->m, c->st);
-    launch_flux(c->P, c->s, c->m, stage, c->st);
+    const Batch B = c->ts.view();
+    launch_eos(B, c->G, 0, c->st);  // primitiveplusbc, interior
+    launch_bc(B, c->G, 0, c->st);   // primitiveplusbc, ghosts
+    launch_grad(B, c->G, c->st);    // myorder 2 only
+    launch_flux(B, c->G, stage, c->st);
 }
 
 int hp_flux(hp_ctx *c)
 {
     if (!c) return fail(nullptr, HP_ERR_ARG, "null context");
+    if (int rc = sync_owner(c)) return rc;
     flux_solver(c, 0);
-    return check_launch(c, "flux");
+    return finish(c, "flux");
 }
 
 int hp_rk_stage(hp_ctx *c, int stage)
 {
     if (!c || stage < 1 || stage > 4) return fail(c, HP_ERR_ARG, "hp_rk_stage: stage must be 1..4");
-    launch_rk_update(c->s, c->m, stage, c->st);
-    return check_launch(c, "rk_stage");
+    if (int rc = sync_owner(c)) return rc;
+    launch_rk_update(c->ts.view(), c->G, stage, c->st);
+    return finish(c, "rk_stage");
 }
 
 int hp_rk(hp_ctx *c)
 {
     if (!c) return fail(nullptr, HP_ERR_ARG, "null context");
-    launch_bc(c->P, c->s, c->m, 1, c->st);
-    launch_viscous(c->P, c->s, c->m, c->st);
+    if (int rc = sync_owner(c)) return rc;
+    launch_bc(c->ts.view(), c->G, 1, c->st);
+    launch_viscous(c->ts.view(), c->G, c->st);
     for (int st = 1; st <= 4; st++) {
         flux_solver(c, 0);
-        launch_rk_update(c->s, c->m, st, c->st);
+        launch_rk_update(c->ts.view(), c->G, st, c->st);
     }
-    return check_launch(c, "rk");
+    return finish(c, "rk");
 }
 
 int hp_end_step(hp_ctx *c, double *diffs)
 {
     if (!c) return fail(nullptr, HP_ERR_ARG, "null context");
-    launch_eos(c->P, c->s, 1, nullptr, c->st);
-    launch_end_step(c->s, diffs ? c->endpart : nullptr, c->st);
+    if (int rc = sync_owner(c)) return rc;
+    launch_eos(c->ts.view(), c->G, 0, c->st);
+    launch_end_step(c->ts.view(), c->G, diffs ? 1 : 0, c->st);
     if (int rc = check_launch(c, "end_step")) return rc;
     if (diffs) {
-        std::vector<double> part(4 * (size_t)c->nendparts);
-        HIPCHK(c, hipMemcpyAsync(part.data(), c->endpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->st));
+        std::vector<double> part(4 * (size_t)c->ts.nct);
+        HIPCHK(c, hipMemcpyAsync(part.data(), c->ts.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         double d[4] = {0, 0, 0, 0};
-        for (int b = 0; b < c->nendparts; b++)
+        for (int b = 0; b < c->ts.nct; b++)
             for (int k = 0; k < 4; k++) d[k] = std::max(d[k], part[(size_t)4 * b + k]);
         // main.F90:653 prints diffu, diffv, diffw, diffp = variables 2,3,4,1
         diffs[0] = d[1]; diffs[1] = d[2]; diffs[2] = d[3]; diffs[3] = d[0];
@@ -408,29 +585,16 @@ int hp_step(hp_ctx *c, int nsteps)
         HIPCHK(c, hipEventCreate(&c->t0));
         HIPCHK(c, hipEventCreate(&c->t1));
     }
+    if (int rc = sync_owner(c)) return rc;
     HIPCHK(c, hipEventRecord(c->t0, c->st));
     int ev = 0;
     for (int n = 0; n < nsteps; n++) {
-        // main.F90:515-558: primitive, total+1, dt = cfl*min(...), time+dt
-        launch_eos(c->P, c->s, 1, c->dtpart, c->st);
-        launch_dt_final(c->P, c->s, c->dtpart, c->nparts, c->st);
-        launch_smooth(c->P, c->s, c->m, c->st);          // :563
-        launch_bc(c->P, c->s, c->m, 1, c->st);           // sr_RK.F90:19
-        launch_viscous(c->P, c->s, c->m, c->st);
-        for (int st = 1; st <= 4; st++) {
-            launch_eos(c->P, c->s, 1, nullptr, c->st);
-            launch_bc(c->P, c->s, c->m, 0, c->st);
-            if (c->P.myorder == 2) launch_grad(c->P, c->s, c->m, c->st);
-            HIPCHK(c, hipEventRecord(c->fev[ev++], c->st));
-            launch_flux(c->P, c->s, c->m, st, c->st);
-            HIPCHK(c, hipEventRecord(c->fev[ev++], c->st));
-        }
-        launch_eos(c->P, c->s, 1, nullptr, c->st);       // main.F90:595
-        launch_end_step(c->s, nullptr, c->st);           // :619-625
+        enqueue_step(c->ts.view(), c->G, c->st, &c->fev[ev]);
+        ev += 8;
     }
     HIPCHK(c, hipEventRecord(c->t1, c->st));
     c->nflux_timed = ev / 2;
-    return check_launch(c, "step");
+    return finish(c, "step");
 }
 
 int hp_last_timing(hp_ctx *c, double *total_ms, double *flux_ms, int *flux_launches)
@@ -512,6 +676,227 @@ int hp_set_array(hp_ctx *c, int which, const double *host, size_t count)
     HIPCHK(c, hipMemcpyAsync(p, host, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return HP_OK;
+}
+
+static int sync_owner(hp_ctx *c)
+{
+    if (c->batch) HIPCHK(c, hipStreamSynchronize(c->batch->st));
+    return HP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches: many cases advanced by the same launches
+// ---------------------------------------------------------------------------
+static int bfail(hp_batch *b, int code, const std::string &msg)
+{
+    if (b) b->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+#define BHIPCHK(b, call)                                                                          \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return bfail((b), HP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+static void batch_grids(hp_batch *b)
+{
+    b->G.cell = b->grid_cell ? round8(b->grid_cell) : round8(b->ts.ct_cap);
+    b->G.quad = b->grid_quad ? round8(b->grid_quad) : round8(b->ts.ct_cap);
+    b->G.ghost = b->grid_ghost ? round8(b->grid_ghost) : round8(b->ts.gt_cap);
+    b->G.slots = b->ts.slots;
+    b->G.single = 0;
+    // a grid with fewer blocks than the tables can hold tiles loops
+    b->G.persist = (b->G.cell < b->ts.ct_cap ? 1 : 0) | (b->G.quad < b->ts.ct_cap ? 2 : 0) |
+                   (b->G.ghost < b->ts.gt_cap ? 4 : 0);
+}
+
+static void batch_drop_graph(hp_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int hp_batch_create(hp_batch **out, int max_cases, int max_cells, int max_ghosts)
+{
+    if (!out) return bfail(nullptr, HP_ERR_ARG, "hp_batch_create: null pointer");
+    *out = nullptr;
+    if (max_cases < 1 || max_cells < 1 || max_ghosts < 0)
+        return bfail(nullptr, HP_ERR_ARG, "hp_batch_create: bad capacity");
+    hp_batch *b = new hp_batch();
+    hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->e0);
+    if (e == hipSuccess) e = hipEventCreate(&b->e1);
+    if (e != hipSuccess) {
+        std::string msg = std::string("hp_batch_create: ") + hipGetErrorString(e);
+        hp_batch_destroy(b);
+        return bfail(nullptr, HP_ERR_HIP, msg);
+    }
+    auto alloc = [b](auto **p, size_t n) -> int {
+        void *d = nullptr;
+        const size_t bytes = std::max<size_t>(n, 1) * sizeof(**p);
+        if (hipMalloc(&d, bytes) != hipSuccess) return bfail(b, HP_ERR_NOMEM, "hp_batch_create: hipMalloc");
+        b->bufs.push_back(d);
+        if (hipMemsetAsync(d, 0, bytes, b->st) != hipSuccess) return bfail(b, HP_ERR_HIP, "hp_batch_create: memset");
+        *p = static_cast<std::remove_reference_t<decltype(*p)>>(d);
+        return HP_OK;
+    };
+    // every case can add one partial tile of cells and one of ghosts
+    const int ct_cap = tiles_of(max_cells) + max_cases, gt_cap = tiles_of(max_ghosts) + max_cases;
+    if (int rc = ts_alloc(b->ts, max_cases, ct_cap, gt_cap, alloc)) {
+        std::string msg = b->err;
+        hp_batch_destroy(b);
+        return bfail(nullptr, rc, msg);
+    }
+    if (int rc = ts_upload(b->ts, b->st, b->err)) {
+        std::string msg = b->err;
+        hp_batch_destroy(b);
+        return bfail(nullptr, rc, msg);
+    }
+    b->members.assign(max_cases, nullptr);
+    batch_grids(b);
+    *out = b;
+    return HP_OK;
+}
+
+int hp_batch_destroy(hp_batch *b)
+{
+    if (!b) return HP_OK;
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    for (hp_ctx *c : b->members)
+        if (c) c->batch = nullptr;
+    batch_drop_graph(b);
+    for (void *p : b->bufs) (void)hipFree(p);
+    if (b->e0) (void)hipEventDestroy(b->e0);
+    if (b->e1) (void)hipEventDestroy(b->e1);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+    return HP_OK;
+}
+
+int hp_batch_add(hp_batch *b, hp_ctx *c, int *slot)
+{
+    if (!b || !c) return bfail(b, HP_ERR_ARG, "hp_batch_add: null argument");
+    if (c->batch) return bfail(b, HP_ERR_ARG, "hp_batch_add: the case already belongs to a batch");
+    if (b->ncases > 0 && (c->P.precision != b->G.precision || c->P.myflux != b->G.myflux ||
+                          c->P.myorder != b->G.myorder))
+        return bfail(b, HP_ERR_ARG, "hp_batch_add: precision, myflux and myorder must match the batch");
+    int k = 0;
+    while (k < b->ts.slots && b->members[k]) k++;
+    if (k == b->ts.slots) return bfail(b, HP_ERR_NOMEM, "hp_batch_add: no free case slot");
+    if (b->ts.nct + tiles_of(c->m.ne) > b->ts.ct_cap || b->ts.ngt + tiles_of(c->m.ng) > b->ts.gt_cap)
+        return bfail(b, HP_ERR_NOMEM, "hp_batch_add: batch cell or ghost capacity exceeded");
+    BHIPCHK(b, hipStreamSynchronize(c->st));
+    BHIPCHK(b, hipStreamSynchronize(b->st));
+    if (b->ncases == 0) {
+        if (b->G.precision != c->P.precision || b->G.myflux != c->P.myflux || b->G.myorder != c->P.myorder)
+            batch_drop_graph(b);
+        b->G.precision = c->P.precision;
+        b->G.myflux = c->P.myflux;
+        b->G.myorder = c->P.myorder;
+    }
+    b->ts.hP[k] = c->P;
+    b->ts.hS[k] = c->s;
+    b->ts.hM[k] = c->m;
+    b->ts.used[k] = 1;
+    if (int rc = ts_upload(b->ts, b->st, b->err)) {
+        b->ts.used[k] = 0;
+        return rc;
+    }
+    b->members[k] = c;
+    b->ncases++;
+    c->batch = b;
+    if (slot) *slot = k;
+    return HP_OK;
+}
+
+int hp_batch_remove(hp_batch *b, hp_ctx *c)
+{
+    if (!b || !c) return bfail(b, HP_ERR_ARG, "hp_batch_remove: null argument");
+    int k = 0;
+    while (k < b->ts.slots && b->members[k] != c) k++;
+    if (k == b->ts.slots) return bfail(b, HP_ERR_ARG, "hp_batch_remove: the case is not in this batch");
+    BHIPCHK(b, hipStreamSynchronize(b->st));
+    b->ts.used[k] = 0;
+    b->members[k] = nullptr;
+    b->ncases--;
+    c->batch = nullptr;
+    return ts_upload(b->ts, b->st, b->err);
+}
+
+int hp_batch_set_grid(hp_batch *b, int cell_blocks, int quad_blocks, int ghost_blocks)
+{
+    if (!b || cell_blocks < 0 || quad_blocks < 0 || ghost_blocks < 0)
+        return bfail(b, HP_ERR_ARG, "hp_batch_set_grid: bad argument");
+    b->grid_cell = cell_blocks;
+    b->grid_quad = quad_blocks;
+    b->grid_ghost = ghost_blocks;
+    batch_grids(b);
+    batch_drop_graph(b);
+    return HP_OK;
+}
+
+int hp_batch_step(hp_batch *b, int nsteps, int use_graph)
+{
+    if (!b || nsteps < 0) return bfail(b, HP_ERR_ARG, "hp_batch_step: bad argument");
+    BHIPCHK(b, hipEventRecord(b->e0, b->st));
+    if (b->ncases > 0 && nsteps > 0) {
+        const Batch B = b->ts.view();
+        if (use_graph) {
+            if (!b->exec) {
+                BHIPCHK(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+                enqueue_step(B, b->G, b->st, nullptr);
+                hipGraph_t g = nullptr;
+                BHIPCHK(b, hipStreamEndCapture(b->st, &g));
+                b->graph = g;
+                BHIPCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            }
+            for (int n = 0; n < nsteps; n++) BHIPCHK(b, hipGraphLaunch(b->exec, b->st));
+        } else {
+            for (int n = 0; n < nsteps; n++) enqueue_step(B, b->G, b->st, nullptr);
+        }
+    }
+    BHIPCHK(b, hipEventRecord(b->e1, b->st));
+    BHIPCHK(b, hipGetLastError());
+    return HP_OK;
+}
+
+int hp_batch_synchronize(hp_batch *b)
+{
+    if (!b) return bfail(nullptr, HP_ERR_ARG, "null batch");
+    BHIPCHK(b, hipStreamSynchronize(b->st));
+    BHIPCHK(b, hipGetLastError());
+    return HP_OK;
+}
+
+int hp_batch_last_ms(hp_batch *b, double *ms)
+{
+    if (!b || !ms) return bfail(b, HP_ERR_ARG, "hp_batch_last_ms: bad argument");
+    BHIPCHK(b, hipStreamSynchronize(b->st));
+    float t = 0.0f;
+    BHIPCHK(b, hipEventElapsedTime(&t, b->e0, b->e1));
+    *ms = t;
+    return HP_OK;
+}
+
+const char *hp_batch_last_error(const hp_batch *b) { return b ? b->err.c_str() : g_create_err.c_str(); }
+
+size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision)
+{
+    if (nodes < 0 || neles < 0 || nghosts < 0) return 0;
+    const size_t ne = neles, ng = nghosts, nt = ne + ng, D = sizeof(double), I = sizeof(int);
+    size_t b = 3 * (size_t)nodes * D;                        // coordinates
+    b += (13 * ne + 4 * ng) * I;                             // nod, nc, ncp, iblank, ghost lists
+    b += (12 + 3 + 12 + 12) * ne * D + 4 * nt * D;         // sc, dl/epssq/dd, fc0, fc3, xc..vol
+    b += (30 + 3 * G_N + 1 + 8 + 10 + 10 + 2 + 10) * ne * D + (size_t)Q_N * nt * D + SC_N * D + I;
+    if (precision == 1) b += (3 * G_N + 1) * ne * sizeof(float);
+    const size_t tc = (ne + TILE - 1) / TILE, tg = (ng + TILE - 1) / TILE + 1;
+    b += sizeof(Params) + sizeof(State) + sizeof(Mesh) + (tc + tg) * sizeof(Tile) + 4 * I + 4 * tc * D;
+    return b + 64 * 4096;  // allocation granularity of the ~60 buffers
 }
 
 }  // extern "C"

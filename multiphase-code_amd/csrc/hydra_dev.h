@@ -51,18 +51,59 @@ struct State {
     int *itot;              // total (step counter used by the outflow BC)
 };
 
+// Work of one launch.  Cells (and boundary ghosts) of every case are cut
+// into tiles of TILE consecutive ids; a tile belongs to one case slot, so a
+// block reads that case's parameters, state and mesh through block-uniform
+// (scalar) loads.  Tile counts live in device memory: a launch (or a
+// captured graph) with a fixed grid serves whatever set of cases the tables
+// describe when it runs.
+constexpr int TILE = 64;
+struct Tile {
+    int cs;     // case slot
+    int first;  // first cell (or ghost) of the case in this tile, 0-based
+    int count;  // 1..TILE
+    int pad;
+};
+struct Batch {
+    const Params *P;        // per slot
+    const State *S;         // per slot
+    const Mesh *M;          // per slot
+    const Tile *ct, *gt;    // cell tiles, ghost tiles
+    const int *cnt;         // [0] cell tiles, [1] ghost tiles
+    const int *slot_t0;     // per slot: its cell tiles are [slot_t0, slot_t1)
+    const int *slot_t1;
+    double *part;           // 4 per cell tile: block partials (dt minimum, diffs)
+    // a context on its own (Grid::single): its case, carried in the launch
+    // arguments, and its cell and ghost counts
+    Params P1;
+    State S1;
+    Mesh M1;
+    int n1, n1g;
+};
+// grid of each launch family (blocks; the tile loops take multiples of 8)
+// and the kernel variants shared by every case of the batch
+struct Grid {
+    int cell;   // per-cell kernels, TILE threads per block
+    int quad;   // four lanes per cell, 4*TILE threads per block
+    int ghost;  // ghost kernels, TILE threads per block
+    int slots;  // k_dt_final, one block per slot
+    int persist;  // bits 1 cell, 2 quad, 4 ghost: the grid may hold fewer blocks than tiles
+    int single;   // one case, taken from Batch::P1/S1/M1
+    int precision, myflux, myorder;
+};
+
 void launch_geometry(const Mesh &m, double xmax, double ymax, double zmax, double ymin, double zmin,
                      hipStream_t st);
-void launch_eos(const Params &P, const State &s, int passes, double *dtpart, hipStream_t st);
-void launch_dt_final(const Params &P, const State &s, const double *dtpart, int nparts, hipStream_t st);
-void launch_bc(const Params &P, const State &s, const Mesh &m, int viscous, hipStream_t st);
-void launch_smooth(const Params &P, const State &s, const Mesh &m, hipStream_t st);
-void launch_viscous(const Params &P, const State &s, const Mesh &m, hipStream_t st);
-void launch_grad(const Params &P, const State &s, const Mesh &m, hipStream_t st);
-void launch_flux(const Params &P, const State &s, const Mesh &m, int stage, hipStream_t st);
-void launch_rk_update(const State &s, const Mesh &m, int stage, hipStream_t st);
-void launch_end_step(const State &s, double *part, hipStream_t st);
-void launch_dt_from_prims(const Params &P, const State &s, double *dtpart, hipStream_t st);
+void launch_eos(const Batch &B, const Grid &G, int want_dt, hipStream_t st);
+void launch_dt_final(const Batch &B, const Grid &G, hipStream_t st);
+void launch_bc(const Batch &B, const Grid &G, int viscous, hipStream_t st);
+void launch_smooth(const Batch &B, const Grid &G, hipStream_t st);
+void launch_viscous(const Batch &B, const Grid &G, hipStream_t st);
+void launch_grad(const Batch &B, const Grid &G, hipStream_t st);
+void launch_flux(const Batch &B, const Grid &G, int stage, hipStream_t st);
+void launch_rk_update(const Batch &B, const Grid &G, int stage, hipStream_t st);
+void launch_end_step(const Batch &B, const Grid &G, int want_diffs, hipStream_t st);
+void launch_dt_from_prims(const Batch &B, const Grid &G, hipStream_t st);
 
 }  // namespace hydra
 

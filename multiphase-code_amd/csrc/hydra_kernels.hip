@@ -52,14 +52,54 @@ __device__ __forceinline__ double fvenkat(double delp, double delm, double epssq
 #define QV(s, i) q[(size_t)(s) * nt + (i)]
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2 cache.  The gather kernels renumber their blocks so that XCD x runs the
-// contiguous logical range [x*nb/8, (x+1)*nb/8): cells that are neighbours in
-// memory order then meet in one L2.  Their grids are a multiple of 8 blocks
-// (launch with nblk8); logical blocks past the last cell do nothing.
-__device__ __forceinline__ int xcd_block()
+// L2 cache.  A tile loop gives the blocks of XCD x (blockIdx % 8 == x) the
+// x-th contiguous eighth of the tile table, so tiles that are neighbours in
+// memory order meet in one L2; within its eighth an XCD's blocks stride by
+// gridDim/8.  Grids are multiples of 8 blocks.
+__device__ __forceinline__ void tile_span(int n, int &t, int &end, int &step)
 {
-    const unsigned b = blockIdx.x, per = gridDim.x >> 3;
-    return (int)((b & 7u) * per + (b >> 3));
+    const int x = (int)(blockIdx.x & 7u);
+    step = (int)(gridDim.x >> 3);
+    t = (int)(((long long)n * x) >> 3) + (int)(blockIdx.x >> 3);
+    end = (int)(((long long)n * (x + 1)) >> 3);
+}
+// MODE (a template parameter of every tile kernel): 0 tables, grid with a
+// block for every tile the tables can hold (at most one tile per block);
+// 1 tables, persistent grid (blocks loop over tiles); 2 one case carried in
+// the launch arguments (B.P1, B.S1, B.M1), a block per tile, no table reads.
+#define TILE_LOOP(NT, N1)                                                    \
+    int t_ = 0, e_ = 0, st_ = 1;                                             \
+    tile_span(MODE == 2 ? ((N1) + TILE - 1) / TILE : (NT), t_, e_, st_);     \
+    for (int t = t_; t < e_; t = (MODE == 1) ? t + st_ : e_)
+
+// the case of tile t and its cell (or ghost) range; by value, so stores
+// through the state pointers cannot alias it
+struct TileCase {
+    Params P;
+    State S;
+    Mesh M;
+    int first, count;
+};
+
+template <int MODE>
+__device__ __forceinline__ TileCase tile_case(const Batch &B, const Tile *tab, int t, int n1)
+{
+    TileCase c;
+    if (MODE == 2) {
+        c.P = B.P1;
+        c.S = B.S1;
+        c.M = B.M1;
+        c.first = t * TILE;
+        c.count = n1 - c.first < TILE ? n1 - c.first : TILE;
+    } else {
+        const Tile T = tab[t];
+        c.P = B.P[T.cs];
+        c.S = B.S[T.cs];
+        c.M = B.M[T.cs];
+        c.first = T.first;
+        c.count = T.count;
+    }
+    return c;
 }
 
 // lanes 4c..4c+3 of a wave cooperate on cell c: one face each
@@ -263,57 +303,17 @@ __device__ __forceinline__ double wave_min(double v)
     return v;
 }
 
-__global__ void __launch_bounds__(256) k_eos(Params P, State s, int passes, double *dtpart)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double loc = 1.0e8;
-    if (i < s.ne) {
-        for (int k = 0; k < passes; k++)
-            eos_cell(P, s.cn, s.ne, i, s.q, s.nt, (dtpart && k == passes - 1) ? &loc : nullptr, s.dl);
-    }
-    if (dtpart) {
-        __shared__ double red[4];
-        double v = wave_min(loc);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double r = red[0];
-            for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = dmin(r, red[w]);
-            dtpart[blockIdx.x] = r;
-        }
-    }
-}
 
-// dtmin over the block minima; dt = cfl*dtmin, time += dt, total += 1
-// (main.F90:520-523, :555-556).  One block of 256 threads.
-__global__ void __launch_bounds__(256) k_dt_final(Params P, State s, const double *dtpart, int nparts)
-{
-    double v = 1.0e8;
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) v = dmin(v, dtpart[b]);
-    __shared__ double red[4];
-    v = wave_min(v);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = red[0];
-        for (int w = 1; w < 4; w++) r = dmin(r, red[w]);
-        double dt = r * P.cfl;
-        s.scal[SC_DTMIN] = r;
-        s.scal[SC_DT] = dt;
-        s.scal[SC_TIME] = s.scal[SC_TIME] + dt;
-        s.itot[0] = s.itot[0] + 1;
-    }
-}
+
+
 
 // ---------------------------------------------------------------------------
 // boundary ghosts (sr_primitiveplusbc.F90:198-592; viscous variant
 // sr_viscousgp.F90:71-446).  Each ghost owns its slot, so the five
 // reference loops collapse into one launch with a switch on the type.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_bc(Params P, State s, Mesh m, int viscous)
+__device__ __forceinline__ void bc_ghost(const Params &P, const State &s, const Mesh &m, int g, int viscous)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= m.ng) return;
     const int ne = m.ne, nt = s.nt, np = m.gpar[g], ng = ne + g, type = m.gtype[g];
     double *q = s.q;
     const double dt = s.scal[SC_DT];
@@ -434,9 +434,8 @@ __device__ __forceinline__ double switch_sum(const double *a, int i, const int *
     return n;
 }
 
-__global__ void __launch_bounds__(256) k_smooth1(State s, Mesh m)
+__device__ __forceinline__ void smooth1_cell(const State &s, const Mesh &m, int i)
 {
-    const int i = xcd_block() * blockDim.x + threadIdx.x;
     const int ne = m.ne;
     if (i >= ne) return;
     int nb[4];
@@ -465,9 +464,8 @@ __global__ void __launch_bounds__(256) k_smooth1(State s, Mesh m)
     }
 }
 
-__global__ void __launch_bounds__(256) k_smooth2(Params P, State s, Mesh m)
+__device__ __forceinline__ void smooth2_cell(const Params &P, const State &s, const Mesh &m, int i)
 {
-    const int i = xcd_block() * blockDim.x + threadIdx.x;
     const int ne = m.ne;
     if (i >= ne) return;
     int nb[4];
@@ -507,13 +505,11 @@ __global__ void __launch_bounds__(256) k_smooth2(Params P, State s, Mesh m)
 // One cell per quad, one face per lane; the quad sums the four face
 // contributions in face order, vis = (((0+V1)+V2)+V3)+V4, as the reference's
 // accumulation into zeroed arrays (sr_viscousgp.F90:41-48, :797-804).
-__global__ void __launch_bounds__(256) k_viscous(Params P, State s, Mesh m)
+__device__ __forceinline__ void viscous_face(const Params &P, const State &s, const Mesh &m, int i, int f,
+                                             int qbase)
 {
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    const int i = t >> 2, f = t & 3;
     const int ne = m.ne;
-    if (i >= ne || m.iblank[i] != 1) return;
-    const int qbase = (int)(threadIdx.x & 63) & ~3;
+    if (m.iblank[i] != 1) return;
     const double *q = s.q;
     const size_t nt = s.nt;
     const int slot[8] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP};
@@ -612,11 +608,10 @@ __global__ void __launch_bounds__(256) k_viscous(Params P, State s, Mesh m)
 // the 4 face neighbours, Cramer's rule) fused with the Venkatakrishnan
 // limiter (sr_VK.F90:32-274), which only needs this cell's gradients.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(128) k_grad_limiter(State s, Mesh m)
+__device__ __forceinline__ void grad_cell(const State &s, const Mesh &m, int i)
 {
-    const int i = xcd_block() * blockDim.x + threadIdx.x;
     const int ne = m.ne;
-    if (i >= ne || m.iblank[i] != 1) return;
+    if (m.iblank[i] != 1) return;
     const double *q = s.q;
     const size_t nt = s.nt;
     int nb[4];
@@ -643,6 +638,7 @@ __global__ void __launch_bounds__(128) k_grad_limiter(State s, Mesh m)
     // rog/rop (and pg/pp) feed the limiter below
     const int slot[13] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP, Q_PG, Q_PP, Q_PHIG, Q_ROG, Q_ROP};
     double glim[4][3];  // pg, pp, rog, rop
+#pragma unroll
     for (int sI = 0; sI < 13; sI++) {
         const int sl = slot[sI];
         const double qi = QV(sl, i);
@@ -1026,14 +1022,12 @@ __device__ __forceinline__ void source_and_update(const Params &P, const State &
 }
 
 template <int FLUX>
-__global__ void __launch_bounds__(256) k_flux(Params P, State s, Mesh m, int stage)
+__device__ __forceinline__ void flux_face(const Params &P, const State &s, const Mesh &m, int i, int f, int qbase,
+                                          int stage)
 {
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    const int i = t >> 2, f = t & 3;
     const int ne = m.ne;
     // all four lanes of a quad share i, so a quad leaves together
-    if (i >= ne || m.iblank[i] != 1) return;
-    const int qbase = (int)(threadIdx.x & 63) & ~3;
+    if (m.iblank[i] != 1) return;
     const double *q = s.q;
     const size_t nt = s.nt;
     const double *s_grad = s.grad;
@@ -1098,11 +1092,10 @@ __global__ void __launch_bounds__(256) k_flux(Params P, State s, Mesh m, int sta
 }
 
 // stage update from a stored residual (sr_RK.F90), for the per-routine path
-__global__ void __launch_bounds__(256) k_rk_update(State s, Mesh m, int stage)
+__device__ __forceinline__ void rk_cell(const State &s, const Mesh &m, int i, int stage)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int ne = m.ne;
-    if (i >= ne || m.iblank[i] != 1) return;
+    if (m.iblank[i] != 1) return;
     const double dtbyvol = s.scal[SC_DT] / m.vol[i];
     for (int j = 0; j < 10; j++) {
         const size_t o = (size_t)j * ne + i;
@@ -1124,33 +1117,16 @@ __global__ void __launch_bounds__(256) k_rk_update(State s, Mesh m, int stage)
     }
 }
 
-// end of step: max|cn-cu| of variables 1..4 per block (main.F90:603-612,
-// a max is order independent) and cu = cn (:619-625)
-__global__ void __launch_bounds__(256) k_end_step(State s, int ne, double *part)
+// end of step: |cn-cu| of variables 1..4 for the monitor maximum
+// (main.F90:603-612, a max is order independent) and cu = cn (:619-625)
+__device__ __forceinline__ void end_cell(const State &s, int i, double *d)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double d[4] = {0.0, 0.0, 0.0, 0.0};
-    if (i < ne) {
-        for (int k = 0; k < 10; k++) {
-            const size_t o = (size_t)k * ne + i;
-            const double c = s.cn[o];
-            if (k < 4) d[k] = fabs(c - s.cu[o]);
-            s.cu[o] = c;
-        }
-    }
-    if (part) {
-        __shared__ double red[4][4];
-        for (int k = 0; k < 4; k++) {
-            double v = d[k];
-            for (int off = 32; off > 0; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
-            if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
-        }
-        __syncthreads();
-        if (threadIdx.x < 4) {
-            double r = red[threadIdx.x][0];
-            for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = dmax(r, red[threadIdx.x][w]);
-            part[(size_t)blockIdx.x * 4 + threadIdx.x] = r;
-        }
+    const int ne = s.ne;
+    for (int k = 0; k < 10; k++) {
+        const size_t o = (size_t)k * ne + i;
+        const double c = s.cn[o];
+        if (k < 4) d[k] = fabs(c - s.cu[o]);
+        s.cu[o] = c;
     }
 }
 
@@ -1158,8 +1134,6 @@ __global__ void __launch_bounds__(256) k_end_step(State s, int ne, double *part)
 // launch helpers
 // ---------------------------------------------------------------------------
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
-// grid for a kernel indexed by xcd_block(): a multiple of 8 blocks
-static inline int nblk8(int n, int b) { return (nblk(n, b) + 7) & ~7; }
 
 void launch_geometry(const Mesh &m, double xmax, double ymax, double zmax, double ymin, double zmin,
                      hipStream_t st)
@@ -1169,108 +1143,22 @@ void launch_geometry(const Mesh &m, double xmax, double ymax, double zmax, doubl
         hipLaunchKernelGGL(k_geom_ghost, dim3(nblk(m.ng, 128)), dim3(128), 0, st, m, xmax, ymax, zmax, ymin, zmin);
 }
 
-void launch_eos(const Params &P, const State &s, int passes, double *dtpart, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_eos, dim3(nblk(s.ne, 256)), dim3(256), 0, st, P, s, passes, dtpart);
-}
-
-void launch_dt_final(const Params &P, const State &s, const double *dtpart, int nparts, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, st, P, s, dtpart, nparts);
-}
-
-void launch_bc(const Params &P, const State &s, const Mesh &m, int viscous, hipStream_t st)
-{
-    if (m.ng > 0) hipLaunchKernelGGL(k_bc, dim3(nblk(m.ng, 256)), dim3(256), 0, st, P, s, m, viscous);
-}
-
-void launch_smooth(const Params &P, const State &s, const Mesh &m, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_smooth1, dim3(nblk8(m.ne, 256)), dim3(256), 0, st, s, m);
-    hipLaunchKernelGGL(k_smooth2, dim3(nblk8(m.ne, 256)), dim3(256), 0, st, P, s, m);
-}
-
-// the FP32-bulk kernels are defined at the end of this file
-__global__ void __launch_bounds__(256) k_viscous_f32(Params P, State s, Mesh m);
-__global__ void __launch_bounds__(128) k_grad_limiter_f32(State s, Mesh m);
-template <int FLUX> __global__ void __launch_bounds__(256) k_flux_f32(Params P, State s, Mesh m, int stage);
-
-void launch_viscous(const Params &P, const State &s, const Mesh &m, hipStream_t st)
-{
-    if (P.precision == 1)
-        hipLaunchKernelGGL(k_viscous_f32, dim3(nblk8(m.ne, 64)), dim3(256), 0, st, P, s, m);
-    else
-        hipLaunchKernelGGL(k_viscous, dim3(nblk8(m.ne, 64)), dim3(256), 0, st, P, s, m);
-}
-
-void launch_grad(const Params &P, const State &s, const Mesh &m, hipStream_t st)
-{
-    if (P.precision == 1)
-        hipLaunchKernelGGL(k_grad_limiter_f32, dim3(nblk8(m.ne, 128)), dim3(128), 0, st, s, m);
-    else
-        hipLaunchKernelGGL(k_grad_limiter, dim3(nblk8(m.ne, 128)), dim3(128), 0, st, s, m);
-}
-
-void launch_flux(const Params &P, const State &s, const Mesh &m, int stage, hipStream_t st)
-{
-    const int nb = nblk8(m.ne, 64);  // 64 cells x 4 faces per block
-    if (P.precision == 1) {
-        if (P.myflux == 1)
-            hipLaunchKernelGGL(k_flux_f32<1>, dim3(nb), dim3(256), 0, st, P, s, m, stage);
-        else
-            hipLaunchKernelGGL(k_flux_f32<2>, dim3(nb), dim3(256), 0, st, P, s, m, stage);
-    } else if (P.myflux == 1) {
-        hipLaunchKernelGGL(k_flux<1>, dim3(nb), dim3(256), 0, st, P, s, m, stage);
-    } else {
-        hipLaunchKernelGGL(k_flux<2>, dim3(nb), dim3(256), 0, st, P, s, m, stage);
-    }
-}
-
-void launch_rk_update(const State &s, const Mesh &m, int stage, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_rk_update, dim3(nblk(m.ne, 256)), dim3(256), 0, st, s, m, stage);
-}
-
-void launch_end_step(const State &s, double *part, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_end_step, dim3(nblk(s.ne, 256)), dim3(256), 0, st, s, s.ne, part);
-}
-
 }  // namespace hydra
 
 namespace hydra {
 
-// CFL bound from the current primitives, one minimum per block
-// (main.F90:525-546)
-__global__ void __launch_bounds__(256) k_dt_from_prims(Params P, State s, double *dtpart)
+// CFL bound of one cell from the current primitives (main.F90:525-546)
+__device__ __forceinline__ double dt_cell(const Params &P, const State &s, int i)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double loc = 1.0e8;
-    if (i < s.ne) {
-        const double *q = s.q;
-        const size_t nt = s.nt;
-        const double ag = sqrt((P.gamag * QV(Q_PG, i)) / QV(Q_ROG, i));
-        const double ap = sqrt((P.gamap * (QV(Q_PP, i) + P.pinf)) / QV(Q_ROP, i));
-        const double ug = QV(Q_UG, i), vg = QV(Q_VG, i), wg = QV(Q_WG, i);
-        const double up = QV(Q_UP, i), vp = QV(Q_VP, i), wp = QV(Q_WP, i);
-        const double qgi = sqrt(ug * ug + vg * vg + wg * wg);
-        const double qpi = sqrt(up * up + vp * vp + wp * wp);
-        loc = s.dl[i] / (dmax(ag, ap) + dmax(qgi, qpi));
-    }
-    __shared__ double red[4];
-    double v = wave_min(loc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = red[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = dmin(r, red[w]);
-        dtpart[blockIdx.x] = r;
-    }
-}
-
-void launch_dt_from_prims(const Params &P, const State &s, double *dtpart, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_dt_from_prims, dim3(nblk(s.ne, 256)), dim3(256), 0, st, P, s, dtpart);
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    const double ag = sqrt((P.gamag * QV(Q_PG, i)) / QV(Q_ROG, i));
+    const double ap = sqrt((P.gamap * (QV(Q_PP, i) + P.pinf)) / QV(Q_ROP, i));
+    const double ug = QV(Q_UG, i), vg = QV(Q_VG, i), wg = QV(Q_WG, i);
+    const double up = QV(Q_UP, i), vp = QV(Q_VP, i), wp = QV(Q_WP, i);
+    const double qgi = sqrt(ug * ug + vg * vg + wg * wg);
+    const double qpi = sqrt(up * up + vp * vp + wp * wp);
+    return s.dl[i] / (dmax(ag, ap) + dmax(qgi, qpi));
 }
 
 }  // namespace hydra
@@ -1312,12 +1200,11 @@ __device__ __forceinline__ float fvenkatf(float delp, float delm, float epssq)
 
 // gradients (sr_derivative.F90:45-322, incl. the cTp2 defect) and limiter
 // (sr_VK.F90:32-274) in FP32, one thread per cell
-__global__ void __launch_bounds__(128) k_grad_limiter_f32(State s, Mesh m)
+__device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int i)
 {
 #pragma clang fp contract(fast)
-    const int i = xcd_block() * blockDim.x + threadIdx.x;
     const int ne = m.ne;
-    if (i >= ne || m.iblank[i] != 1) return;
+    if (m.iblank[i] != 1) return;
     const double *q = s.q;
     const size_t nt = s.nt;
     int nb[4];
@@ -1342,6 +1229,7 @@ __global__ void __launch_bounds__(128) k_grad_limiter_f32(State s, Mesh m)
     const float idn = frcp(det3f(a1, b1, b2, b1, a2, b3, b2, b3, a3));
     const int slot[13] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP, Q_PG, Q_PP, Q_PHIG, Q_ROG, Q_ROP};
     float glim[4][3];
+#pragma unroll
     for (int sI = 0; sI < 13; sI++) {
         const int sl = slot[sI];
         const double qi = QV(sl, i);
@@ -1593,14 +1481,12 @@ __device__ __forceinline__ void hllc_zien32(const Params &P, const FState32 &L, 
 // face fluxes are widened to FP64 and summed in face order, then the FP64
 // source and stage update run unchanged.
 template <int FLUX>
-__global__ void __launch_bounds__(256) k_flux_f32(Params P, State s, Mesh m, int stage)
+__device__ __forceinline__ void flux_face_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
+                                              int qbase, int stage)
 {
 #pragma clang fp contract(fast)
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    const int i = t >> 2, f = t & 3;
     const int ne = m.ne;
-    if (i >= ne || m.iblank[i] != 1) return;
-    const int qbase = (int)(threadIdx.x & 63) & ~3;
+    if (m.iblank[i] != 1) return;
     const double *q = s.q;
     const size_t nt = s.nt;
     const int neigh = m.nc[(size_t)f * ne + i];
@@ -1659,14 +1545,12 @@ __global__ void __launch_bounds__(256) k_flux_f32(Params P, State s, Mesh m, int
 }
 
 // viscous face flux (sr_viscousgp.F90:450-1139) in FP32, one face per lane
-__global__ void __launch_bounds__(256) k_viscous_f32(Params P, State s, Mesh m)
+__device__ __forceinline__ void viscous_face_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
+                                                 int qbase)
 {
 #pragma clang fp contract(fast)
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    const int i = t >> 2, f = t & 3;
     const int ne = m.ne;
-    if (i >= ne || m.iblank[i] != 1) return;
-    const int qbase = (int)(threadIdx.x & 63) & ~3;
+    if (m.iblank[i] != 1) return;
     const double *q = s.q;
     const size_t nt = s.nt;
     const int slot[8] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP};
@@ -1751,5 +1635,262 @@ __global__ void __launch_bounds__(256) k_viscous_f32(Params P, State s, Mesh m)
         if ((v & 3) == f) s.vis[(size_t)v * ne + i] = 0.0 + t0 + t1 + t2 + t3;
     }
 }
+
+}  // namespace hydra
+
+// ===========================================================================
+// Kernels: one launch covers every case of a batch.  Each block walks its
+// share of the tile table (tile_span); per-cell kernels run TILE threads, one
+// cell each, and the face kernels run 4*TILE threads, four lanes per cell.
+// ===========================================================================
+namespace hydra {
+
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// primitive (sr_primitive.F90:18-178) on interior cells; want_dt: also the
+// CFL bound per tile (main.F90:541-545)
+template <int MODE, bool WANT_DT>
+__global__ void __launch_bounds__(TILE) k_eos(Batch B)
+{
+    const bool want_dt = WANT_DT;
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        const State &s = C.S;
+        double loc = 1.0e8;
+        if ((int)threadIdx.x < C.count)
+            eos_cell(C.P, s.cn, s.ne, C.first + (int)threadIdx.x, s.q, s.nt, want_dt ? &loc : nullptr, s.dl);
+        if (want_dt) {
+            const double v = wave_min(loc);
+            if (threadIdx.x == 0) B.part[(size_t)4 * t] = v;
+        }
+    }
+}
+
+// per case slot (one block each): dtmin over its tiles, dt = cfl*dtmin,
+// time += dt, total += 1 (main.F90:520-523, :555-556); a min does not
+// depend on the order it is taken in
+__global__ void __launch_bounds__(1024) k_dt_final(Batch B)
+{
+    const int c = blockIdx.x;
+    const int t0 = B.slot_t0[c], t1 = B.slot_t1[c];
+    if (t0 >= t1) return;
+    double v = 1.0e8;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += (int)blockDim.x) v = dmin(v, B.part[(size_t)4 * t]);
+    __shared__ double red[16];
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = dmin(r, red[w]);
+        const State &s = B.S[c];
+        const double dt = r * B.P[c].cfl;
+        s.scal[SC_DTMIN] = r;
+        s.scal[SC_DT] = dt;
+        s.scal[SC_TIME] = s.scal[SC_TIME] + dt;
+        s.itot[0] = s.itot[0] + 1;
+    }
+}
+
+// CFL bound per tile from the current primitives (hp_dt_min)
+template <int MODE>
+__global__ void __launch_bounds__(TILE) k_dt_from_prims(Batch B)
+{
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        double loc = 1.0e8;
+        if ((int)threadIdx.x < C.count) loc = dt_cell(C.P, C.S, C.first + (int)threadIdx.x);
+        const double v = wave_min(loc);
+        if (threadIdx.x == 0) B.part[(size_t)4 * t] = v;
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(TILE) k_bc(Batch B, int viscous)
+{
+    TILE_LOOP(B.cnt[1], B.n1g)
+    {
+        const TileCase C = tile_case<MODE>(B, B.gt, t, B.n1g);
+        if ((int)threadIdx.x < C.count)
+            bc_ghost(C.P, C.S, C.M, C.first + (int)threadIdx.x, viscous);
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(TILE) k_smooth1(Batch B)
+{
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        if ((int)threadIdx.x < C.count) smooth1_cell(C.S, C.M, C.first + (int)threadIdx.x);
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(TILE) k_smooth2(Batch B)
+{
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        if ((int)threadIdx.x < C.count) smooth2_cell(C.P, C.S, C.M, C.first + (int)threadIdx.x);
+    }
+}
+
+template <int MODE, bool F32>
+__global__ void __launch_bounds__(TILE) k_grad_limiter(Batch B)
+{
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        if ((int)threadIdx.x < C.count) {
+            if (F32)
+                grad_cell_f32(C.S, C.M, C.first + (int)threadIdx.x);
+            else
+                grad_cell(C.S, C.M, C.first + (int)threadIdx.x);
+        }
+    }
+}
+
+// lanes 4l..4l+3 of a block own cell C.first + l, one face each
+template <int MODE, bool F32>
+__global__ void __launch_bounds__(4 * TILE) k_viscous(Batch B)
+{
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        const int l = (int)(threadIdx.x >> 2), f = (int)(threadIdx.x & 3), qbase = (int)(threadIdx.x & 63) & ~3;
+        if (l < C.count) {
+            if (F32)
+                viscous_face_f32(C.P, C.S, C.M, C.first + l, f, qbase);
+            else
+                viscous_face(C.P, C.S, C.M, C.first + l, f, qbase);
+        }
+    }
+}
+
+template <int MODE, int FLUX, bool F32>
+__global__ void __launch_bounds__(4 * TILE) k_flux(Batch B, int stage)
+{
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        const int l = (int)(threadIdx.x >> 2), f = (int)(threadIdx.x & 3), qbase = (int)(threadIdx.x & 63) & ~3;
+        if (l < C.count) {
+            if (F32)
+                flux_face_f32<FLUX>(C.P, C.S, C.M, C.first + l, f, qbase, stage);
+            else
+                flux_face<FLUX>(C.P, C.S, C.M, C.first + l, f, qbase, stage);
+        }
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(TILE) k_rk_update(Batch B, int stage)
+{
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        if ((int)threadIdx.x < C.count) rk_cell(C.S, C.M, C.first + (int)threadIdx.x, stage);
+    }
+}
+
+// cu = cn; want_diffs: the tile's max|cn-cu| of variables 1..4 into part
+template <int MODE>
+__global__ void __launch_bounds__(TILE) k_end_step(Batch B, int want_diffs)
+{
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        double d[4] = {0.0, 0.0, 0.0, 0.0};
+        if ((int)threadIdx.x < C.count) end_cell(C.S, C.first + (int)threadIdx.x, d);
+        if (want_diffs) {
+            for (int k = 0; k < 4; k++) {
+                const double v = wave_max(d[k]);
+                if (threadIdx.x == 0) B.part[(size_t)4 * t + k] = v;
+            }
+        }
+    }
+}
+
+// MODE of a launch: 2 for a context on its own, 1 for a persistent grid
+// (G.persist bit of the family), 0 otherwise
+#define PICK(bit, K, ...) \
+    (G.single ? K<2, ##__VA_ARGS__> : (G.persist & (bit)) ? K<1, ##__VA_ARGS__> : K<0, ##__VA_ARGS__>)
+
+void launch_eos(const Batch &B, const Grid &G, int want_dt, hipStream_t st)
+{
+    if (want_dt)
+        hipLaunchKernelGGL(PICK(1, k_eos, true), dim3(G.cell), dim3(TILE), 0, st, B);
+    else
+        hipLaunchKernelGGL(PICK(1, k_eos, false), dim3(G.cell), dim3(TILE), 0, st, B);
+}
+
+void launch_dt_final(const Batch &B, const Grid &G, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dt_final, dim3(G.slots), dim3(1024), 0, st, B);
+}
+
+void launch_dt_from_prims(const Batch &B, const Grid &G, hipStream_t st)
+{
+    hipLaunchKernelGGL(PICK(1, k_dt_from_prims), dim3(G.cell), dim3(TILE), 0, st, B);
+}
+
+void launch_bc(const Batch &B, const Grid &G, int viscous, hipStream_t st)
+{
+    hipLaunchKernelGGL(PICK(4, k_bc), dim3(G.ghost), dim3(TILE), 0, st, B, viscous);
+}
+
+void launch_smooth(const Batch &B, const Grid &G, hipStream_t st)
+{
+    hipLaunchKernelGGL(PICK(1, k_smooth1), dim3(G.cell), dim3(TILE), 0, st, B);
+    hipLaunchKernelGGL(PICK(1, k_smooth2), dim3(G.cell), dim3(TILE), 0, st, B);
+}
+
+void launch_viscous(const Batch &B, const Grid &G, hipStream_t st)
+{
+    if (G.precision == 1)
+        hipLaunchKernelGGL(PICK(2, k_viscous, true), dim3(G.quad), dim3(4 * TILE), 0, st, B);
+    else
+        hipLaunchKernelGGL(PICK(2, k_viscous, false), dim3(G.quad), dim3(4 * TILE), 0, st, B);
+}
+
+void launch_grad(const Batch &B, const Grid &G, hipStream_t st)
+{
+    if (G.myorder != 2) return;
+    if (G.precision == 1)
+        hipLaunchKernelGGL(PICK(1, k_grad_limiter, true), dim3(G.cell), dim3(TILE), 0, st, B);
+    else
+        hipLaunchKernelGGL(PICK(1, k_grad_limiter, false), dim3(G.cell), dim3(TILE), 0, st, B);
+}
+
+void launch_flux(const Batch &B, const Grid &G, int stage, hipStream_t st)
+{
+    const dim3 g(G.quad), b(4 * TILE);
+    if (G.precision == 1) {
+        if (G.myflux == 1) hipLaunchKernelGGL(PICK(2, k_flux, 1, true), g, b, 0, st, B, stage);
+        else hipLaunchKernelGGL(PICK(2, k_flux, 2, true), g, b, 0, st, B, stage);
+    } else {
+        if (G.myflux == 1) hipLaunchKernelGGL(PICK(2, k_flux, 1, false), g, b, 0, st, B, stage);
+        else hipLaunchKernelGGL(PICK(2, k_flux, 2, false), g, b, 0, st, B, stage);
+    }
+}
+
+void launch_rk_update(const Batch &B, const Grid &G, int stage, hipStream_t st)
+{
+    hipLaunchKernelGGL(PICK(1, k_rk_update), dim3(G.cell), dim3(TILE), 0, st, B, stage);
+}
+
+void launch_end_step(const Batch &B, const Grid &G, int want_diffs, hipStream_t st)
+{
+    hipLaunchKernelGGL(PICK(1, k_end_step), dim3(G.cell), dim3(TILE), 0, st, B, want_diffs);
+}
+
+#undef PICK
 
 }  // namespace hydra

@@ -41,7 +41,10 @@ _lib = None
 EXPORTS = ["hp_create", "hp_destroy", "hp_upload_state", "hp_download_state", "hp_primitive",
            "hp_dt_min", "hp_set_dt", "hp_smooth", "hp_viscous", "hp_flux", "hp_rk_stage", "hp_rk",
            "hp_end_step", "hp_step", "hp_synchronize", "hp_last_error", "hp_get_array",
-           "hp_set_array", "hp_last_timing", "hp_probe_cell"]
+           "hp_set_array", "hp_last_timing", "hp_probe_cell",
+           "hp_batch_create", "hp_batch_destroy", "hp_batch_add", "hp_batch_remove", "hp_batch_set_grid",
+           "hp_batch_step", "hp_batch_synchronize", "hp_batch_last_ms", "hp_batch_last_error",
+           "hp_case_bytes"]
 
 
 def lib():
@@ -72,7 +75,20 @@ def lib():
     L.hp_set_array.argtypes = [vp, c_int, dp, c_size]
     L.hp_last_timing.argtypes = [vp, dp, dp, ip]
     L.hp_probe_cell.argtypes = [vp, c_int, dp]
+    L.hp_batch_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int]
+    L.hp_batch_destroy.argtypes = [vp]
+    L.hp_batch_add.argtypes = [vp, vp, ip]
+    L.hp_batch_remove.argtypes = [vp, vp]
+    L.hp_batch_set_grid.argtypes = [vp, c_int, c_int, c_int]
+    L.hp_batch_step.argtypes = [vp, c_int, c_int]
+    L.hp_batch_synchronize.argtypes = [vp]
+    L.hp_batch_last_ms.argtypes = [vp, dp]
+    L.hp_batch_last_error.argtypes = [vp]
+    L.hp_batch_last_error.restype = ctypes.c_char_p
+    L.hp_case_bytes.argtypes = [c_int, c_int, c_int, c_int]
+    L.hp_case_bytes.restype = c_size
     for n in EXPORTS:
-        getattr(L, n).restype = getattr(L, n).restype if n == "hp_last_error" else c_int
+        if n not in ("hp_last_error", "hp_batch_last_error", "hp_case_bytes"):
+            getattr(L, n).restype = c_int
     _lib = L
     return L

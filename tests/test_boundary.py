@@ -18,7 +18,7 @@ HEADER = os.path.join(ROOT, "include", "hydra_hip.h")
 
 def declared():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s*\*?\s*(hp_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void|size_t)\s*\*?\s*(hp_\w+)\s*\(", txt, re.M)))
 
 
 def test_header_declares_all_bound_symbols():

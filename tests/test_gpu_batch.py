@@ -1,0 +1,151 @@
+"""Batches (hp_batch): many cases advanced by the same launches.
+
+Every cell's arithmetic reads only its own case, so a case's result must be
+bit-for-bit the same alone (hp_step) and in any mix of other cases, with or
+without the captured graph, for any launch shape the tuner may pick, and
+while other cases join and leave.  Each case runs its own dt.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from hydraphase import Solver
+from hydraphase import batch as HB
+from hydraphase import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+def specs():
+    # uneven sizes, both viscosities, different Mach numbers and radii
+    return [HB.CaseSpec(5, 0.3, 0.2, True, 11), HB.CaseSpec(8, 0.6, 0.12, False, 12),
+            HB.CaseSpec(4, 0.15, 0.25, True, 13), HB.CaseSpec(7, 0.45, 0.08, False, 14)]
+
+
+def alone(case, nsteps):
+    m, p, cu, gp, _ = case
+    s = Solver(m, p)
+    s.upload(cu, gp, 0.0, 0)
+    s.step(nsteps)
+    out = s.download()
+    s.close()
+    return out
+
+
+def in_batch(cases, nsteps, graph=True, grid=(0, 0, 0)):
+    b = HB.Batch(*HB.capacity([c[0] for c in cases]))
+    b.set_grid(*grid)
+    sols = []
+    for (m, p, cu, gp, _) in cases:
+        s = Solver(m, p)
+        s.upload(cu, gp, 0.0, 0)
+        b.add(s)
+        sols.append(s)
+    b.step(nsteps, graph=graph)
+    b.synchronize()
+    outs = [s.download() for s in sols]
+    b.close()
+    for s in sols:
+        s.close()
+    return outs
+
+
+def same(a, b):
+    cu_a, gp_a, t_a, n_a = a
+    cu_b, gp_b, t_b, n_b = b
+    return n_a == n_b and t_a == t_b and np.array_equal(cu_a, cu_b) and np.array_equal(gp_a, gp_b)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_equals_alone(precision):
+    cases = [HB.build_case(s, precision) for s in specs()]
+    ref = [alone(c, 4) for c in cases]
+    for graph in (True, False):
+        got = in_batch(cases, 4, graph=graph)
+        for k in range(len(cases)):
+            assert same(got[k], ref[k]), (precision, graph, k)
+
+
+def test_batch_fp64_case_matches_oracle():
+    cases = [HB.build_case(s, 0) for s in specs()[:2]]
+    got = in_batch(cases, 3)
+    for (m, p, cu, gp, _), g in zip(cases, got):
+        o = O.Oracle(m, p)
+        o.set_state(cu, gp, 0.0, 0)
+        o.run(3)
+        cu_o, gp_o, t_o, n_o = o.get_state()
+        assert n_o == g[3] == 3 and t_o == g[2]
+        assert np.array_equal(cu_o, g[0])
+
+
+def test_launch_shapes_give_same_results():
+    cases = [HB.build_case(s, 1) for s in specs()]
+    ref = in_batch(cases, 3)
+    for grid in [(8, 8, 8), (16, 24, 8), (512, 1024, 64)]:
+        got = in_batch(cases, 3, grid=grid)
+        for k in range(len(cases)):
+            assert same(got[k], ref[k]), (grid, k)
+
+
+def test_join_and_leave_with_one_graph():
+    """A: steps 0-5; B joins at step 2 of A and leaves after 2 steps; C
+    joins with B and stays.  Each must equal its alone run."""
+    sp = specs()
+    ca, cb, cc = (HB.build_case(s, 0) for s in sp[:3])
+    b = HB.Batch(*HB.capacity([ca[0], cb[0], cc[0]]))
+    sols = {}
+    for name, c in (("a", ca), ("b", cb), ("c", cc)):
+        s = Solver(c[0], c[1])
+        s.upload(c[2], c[3], 0.0, 0)
+        sols[name] = s
+    b.add(sols["a"])
+    b.step(2)
+    b.add(sols["b"])
+    b.add(sols["c"])
+    b.step(2)
+    b.remove(sols["b"])
+    out_b = sols["b"].download()
+    b.step(1)
+    b.synchronize()
+    out_a = sols["a"].download()
+    out_c = sols["c"].download()
+    b.close()
+    assert same(out_a, alone(ca, 5))
+    assert same(out_b, alone(cb, 2))
+    assert same(out_c, alone(cc, 3))
+
+
+def test_morton_order_same_per_cell_results():
+    sp = specs()[1]
+    nat = HB.build_case(sp, 0, morton=False)
+    mor = HB.build_case(sp, 0, morton=True)
+    a = alone(nat, 3)
+    b = in_batch([mor], 3)[0]
+    perm = mor[4]
+    assert np.array_equal(b[0], a[0][:, perm])
+    assert np.array_equal(b[1], a[1]) and b[2] == a[2]
+
+
+def test_case_bytes_covers_allocation():
+    """hp_case_bytes is what the request loop admits by: the device memory a
+    case really takes must not exceed it by more than allocation rounding."""
+    import ctypes
+    from hydraphase import _lib
+    _lib.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the HIP runtime the library already loaded
+
+    def free_bytes():
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        assert hip.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) == 0
+        return f.value
+
+    m, p, cu, gp, _ = HB.build_case(HB.CaseSpec(32, 0.3, 0.2, True, 21), 1)
+    est = HB.case_bytes(m, 1)
+    free0 = free_bytes()
+    s = Solver(m, p)
+    s.synchronize()
+    free1 = free_bytes()
+    used = free0 - free1
+    s.close()
+    print(f"case bytes: estimate {est / 2**20:.1f} MiB, allocated {used / 2**20:.1f} MiB")
+    assert used <= 1.05 * est + (64 << 20), (used, est)
