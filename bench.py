@@ -150,6 +150,81 @@ def summarize_trace(db_path, top=20):
         print(f"| `{name}` | {r[1]} | {r[2]:.1f} | {r[3]:.1f} | {r[4]:.2f} |")
 
 
+def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
+    """The 64-case uneven ensemble in its FP32-bulk batched form (autotuned
+    launch shape, graph-captured step) against the plain one-case FP64 path
+    (each case alone through hp_step) on the same cases; the error of the
+    batched results against the FP64 path (bitwise the oracle's) for a
+    sample of cases."""
+    from hydraphase import Solver
+    from hydraphase import batch as HB
+    from hydraphase import serve as SV
+    specs = HB.ensemble_specs(64, seed=1, steps=(steps, steps))
+    t0 = time.time()
+    tuning = HB.tune(specs, precision=1, log=log)
+    log(f"# tuned in {time.time() - t0:.1f} s: morton={tuning['morton']} grid={tuning['grid']}")
+    res = HB.run_ensemble(specs, precision=1, tuning=tuning, graph=True)
+    batched = res["updates"] / (res["ms"] * 1e-3)
+    # plain one-case FP64 path: each case alone, baseline_steps steps after 2 warm-up steps
+    upd, ms = 0, 0.0
+    for sp in specs:
+        m, p, cu, gp, _ = HB.build_case(sp, 0)
+        s = Solver(m, p)
+        s.upload(cu, gp, 0.0, 0)
+        s.step(2)
+        s.step(baseline_steps)
+        tot, _, _ = s.last_timing()
+        ms += tot
+        upd += 4 * baseline_steps * int(m.iblank.sum())
+        s.close()
+    plain = upd / (ms * 1e-3)
+    errs = []
+    for k in np.linspace(0, len(specs) - 1, sample).astype(int):
+        m, p, cu, gp, _ = HB.build_case(specs[k], 0)
+        s = Solver(m, p)
+        s.upload(cu, gp, 0.0, 0)
+        s.step(steps)
+        ref = s.download()[0]
+        s.close()
+        got = res["states"][k][0]
+        if res["perms"][k] is not None:
+            un = np.empty_like(got)
+            un[:, res["perms"][k]] = got
+            got = un
+        errs.append(float(SV.mixture_rel_l2(got, ref).max()))
+    cells = sum(int(mm.iblank.sum()) for mm in res["meshes"])
+    return {"workload": f"64 boxes n^3 x 6 tets, n log-uniform 8..48 ({cells} tets), Mach 0.1-0.8, "
+                        f"R 0.05-0.25, half viscous, {steps} steps each",
+            "updates_per_s": batched, "ms_per_step": res["ms"] / steps,
+            "plain_fp64_one_case_updates_per_s": plain, "speedup_vs_plain_fp64": batched / plain,
+            "tuning": {"morton": tuning["morton"], "grid": list(tuning["grid"])},
+            "mixture_rel_l2_vs_fp64_sample": errs, "sample_cases": sample}
+
+
+def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
+    """Poisson arrivals of cases (n 10..70, 50..500 steps), admitted by a
+    200 GB HBM budget and stepped together (FP32 bulk)."""
+    from hydraphase import serve as SV
+    reqs = SV.poisson_requests(n_requests, rate=rate, seed=7, nmin=10, nmax=70, steps=(50, 500), precision=1)
+    srv = SV.Server(budget_bytes=200e9, max_cases=256, chunk=25)
+    try:
+        st = srv.run(reqs)
+    finally:
+        srv.close()
+    k = max(1, int(round(sample_frac * n_requests)))
+    errs = []
+    for r in reqs[:: max(1, n_requests // k)][:k]:
+        ref = SV.alone(r, precision=0)
+        errs.append(float(SV.mixture_rel_l2(r.result[0], ref[0]).max()))
+    cells = sum(int(r.mesh.iblank.sum()) for r in reqs)
+    return {"workload": f"{n_requests} requests, Poisson {rate}/s, boxes n 10..70 ({cells} tets in all), "
+                        "50..500 steps, 200 GB budget, FP32 bulk",
+            "updates_per_s": st["updates_per_s"], "tts_p50_s": st["tts_p50_s"], "tts_p99_s": st["tts_p99_s"],
+            "makespan_s": st["makespan_s"], "gpu_busy_s": st["gpu_busy_s"],
+            "peak_resident_gb": st["peak_resident_bytes"] / 1e9,
+            "mixture_rel_l2_vs_fp64_sample": errs}
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-counters":
         summarize_counters(sys.argv[2], sys.argv[3], int(sys.argv[4]))
@@ -165,6 +240,8 @@ def main():
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1],
                     help="0: FP64 drop-in (the headline); 1: FP32-bulk variant")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched 64-case ensemble")
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-loop run")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run a plain device copy (b=2a, float64) of this many MiB, the "
@@ -211,7 +288,7 @@ def main():
     bpc = FLUX_BYTES_PER_CELL if args.precision == 0 else FLUX_BYTES_PER_CELL_F32
     achieved = bpc * active / (flux_avg_ms * 1e-3) / 1e9
     out = {
-        "metric": "cell-stage updates/s @5M tets",
+        "metric": "cell-stage updates/s @5M tets, %HBM peak; serve p50/p99 solve latency",
         "value": value,
         "unit": "cell-stage updates/s",
         "n_gpus": 1,
@@ -232,6 +309,14 @@ def main():
                      "traffic": flux_traffic_bytes() if list(args.n) == [96, 96, 90] and args.precision == 0 else None,
                      "flux_avg_ms": flux_avg_ms, "bytes_per_cell": bpc},
     }
+    s.close()
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(log)
+        log(f"# ensemble: {out['ensemble']['updates_per_s']:.3e} updates/s")
+    if not args.no_serve:
+        out["serve"] = serve_bench(log)
+        log(f"# serve: p50 {out['serve']['tts_p50_s']:.2f} s p99 {out['serve']['tts_p99_s']:.2f} s")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(m, p, cu, gp, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -147,8 +147,10 @@ int hp_batch_destroy(hp_batch *b);   /* members stay valid, no longer batched */
 int hp_batch_add(hp_batch *b, hp_ctx *ctx, int *slot);
 int hp_batch_remove(hp_batch *b, hp_ctx *ctx);
 /* blocks per launch family (cell kernels, four-lanes-per-cell face kernels,
- * ghost kernels); rounded up to multiples of 8; 0 = one block per tile at
- * capacity.  Chosen by the tuner in hydraphase.batch. */
+ * ghost kernels); rounded up to multiples of 8; a grid smaller than the
+ * tiles loops over them.  0 = one block per tile: per tile of capacity for
+ * graph replays (the graph outlives membership changes), per tile in use
+ * for eager steps.  Chosen by the tuner in hydraphase.batch. */
 int hp_batch_set_grid(hp_batch *b, int cell_blocks, int quad_blocks, int ghost_blocks);
 /* nsteps time steps of every member; use_graph: replay one captured step */
 int hp_batch_step(hp_batch *b, int nsteps, int use_graph);

@@ -857,7 +857,14 @@ int hp_batch_step(hp_batch *b, int nsteps, int use_graph)
             }
             for (int n = 0; n < nsteps; n++) BHIPCHK(b, hipGraphLaunch(b->exec, b->st));
         } else {
-            for (int n = 0; n < nsteps; n++) enqueue_step(B, b->G, b->st, nullptr);
+            // eager launches know the current tile counts: untuned families
+            // get exactly a block per tile
+            Grid G = b->G;
+            if (!b->grid_cell) G.cell = round8(b->ts.nct);
+            if (!b->grid_quad) G.quad = round8(b->ts.nct);
+            if (!b->grid_ghost) G.ghost = round8(b->ts.ngt);
+            G.persist = (G.cell < b->ts.nct ? 1 : 0) | (G.quad < b->ts.nct ? 2 : 0) | (G.ghost < b->ts.ngt ? 4 : 0);
+            for (int n = 0; n < nsteps; n++) enqueue_step(B, G, b->st, nullptr);
         }
     }
     BHIPCHK(b, hipEventRecord(b->e1, b->st));
