@@ -125,6 +125,20 @@ int hp_set_array(hp_ctx *ctx, int which, const double *host, size_t count);
  * context's stream: total milliseconds and the flux kernel's share. */
 int hp_last_timing(hp_ctx *ctx, double *total_ms, double *flux_ms, int *flux_launches);
 
+/* Overset coupling by fixed donor lists.  No counterpart in the reference,
+ * which only freezes iblank cells (its overset preprocessing is an
+ * external library); this serves two overlapping meshes held as one case
+ * (disconnected components of one mesh).  recv(n): receptor cells;
+ * donors(ld,4) and weights(ld,4): column-major, ld >= n; ids 1-based.
+ * After every RK stage update (hp_step, hp_rk, hp_rk_stage) and on
+ * hp_apply_donors, each receptor's ten conserved variables become
+ * w1*cn(d1) + w2*cn(d2) + w3*cn(d3) + w4*cn(d4), summed in that order.
+ * Receptors must not be donors and should have iblank != 1 (the update
+ * skips them); n = 0 removes the lists.  A case with donor lists cannot
+ * join a batch. */
+int hp_set_donors(hp_ctx *ctx, int n, const int *recv, const int *donors, int ld, const double *weights);
+int hp_apply_donors(hp_ctx *ctx);
+
 /* ---------------------------------------------------------------------------
  * Batches: many independent cases advanced by the same kernel launches.
  * No counterpart in the reference (one case per process there); these serve

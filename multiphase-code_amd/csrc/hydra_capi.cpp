@@ -78,6 +78,8 @@ struct hp_ctx {
     TileSet ts;
     Grid G{};
     hp_batch *batch = nullptr;  // the batch this case belongs to, if any
+    Donors D{};                 // overset donor lists (n = 0: none)
+    std::vector<void *> dbufs;  // their device buffers
     std::string err;
     // timing of the last hp_step
     std::vector<hipEvent_t> fev;  // start/stop pairs around flux launches
@@ -196,7 +198,7 @@ int ts_upload(TileSet &t, hipStream_t st, std::string &err)
 
 // one time step (main.F90:515-625) of every case in B; fev (8 events, may
 // be null) brackets the four flux launches
-void enqueue_step(const Batch &B, const Grid &G, hipStream_t st, hipEvent_t *fev)
+void enqueue_step(const Batch &B, const Grid &G, hipStream_t st, hipEvent_t *fev, const Donors *D = nullptr)
 {
     launch_eos(B, G, 1, st);       // primitive + CFL bound per tile
     launch_dt_final(B, G, st);     // dt, time, total per case
@@ -210,6 +212,7 @@ void enqueue_step(const Batch &B, const Grid &G, hipStream_t st, hipEvent_t *fev
         if (fev) (void)hipEventRecord(fev[2 * (k - 1)], st);
         launch_flux(B, G, k, st);  // face flux + source + stage k update
         if (fev) (void)hipEventRecord(fev[2 * (k - 1) + 1], st);
+        if (D) launch_donor_gather(B.S1, *D, st);  // overset receptors of a lone case
     }
     launch_eos(B, G, 0, st);       // main.F90:595
     launch_end_step(B, G, 0, st);  // :619-625
@@ -382,6 +385,8 @@ int hp_destroy(hp_ctx *c)
     if (!c) return HP_OK;
     if (c->batch) (void)hp_batch_remove(c->batch, c);
     if (c->st) (void)hipStreamSynchronize(c->st);
+    for (void *p : c->dbufs) (void)hipFree(p);
+    if (c->st) (void)hipStreamSynchronize(c->st);
     for (void *p : c->bufs) (void)hipFree(p);
     for (hipEvent_t e : c->fev) (void)hipEventDestroy(e);
     if (c->t0) (void)hipEventDestroy(c->t0);
@@ -536,6 +541,7 @@ int hp_rk_stage(hp_ctx *c, int stage)
     if (!c || stage < 1 || stage > 4) return fail(c, HP_ERR_ARG, "hp_rk_stage: stage must be 1..4");
     if (int rc = sync_owner(c)) return rc;
     launch_rk_update(c->ts.view(), c->G, stage, c->st);
+    launch_donor_gather(c->s, c->D, c->st);
     return finish(c, "rk_stage");
 }
 
@@ -548,6 +554,7 @@ int hp_rk(hp_ctx *c)
     for (int st = 1; st <= 4; st++) {
         flux_solver(c, 0);
         launch_rk_update(c->ts.view(), c->G, st, c->st);
+        launch_donor_gather(c->s, c->D, c->st);
     }
     return finish(c, "rk");
 }
@@ -589,7 +596,7 @@ int hp_step(hp_ctx *c, int nsteps)
     HIPCHK(c, hipEventRecord(c->t0, c->st));
     int ev = 0;
     for (int n = 0; n < nsteps; n++) {
-        enqueue_step(c->ts.view(), c->G, c->st, &c->fev[ev]);
+        enqueue_step(c->ts.view(), c->G, c->st, &c->fev[ev], c->D.n ? &c->D : nullptr);
         ev += 8;
     }
     HIPCHK(c, hipEventRecord(c->t1, c->st));
@@ -676,6 +683,63 @@ int hp_set_array(hp_ctx *c, int which, const double *host, size_t count)
     HIPCHK(c, hipMemcpyAsync(p, host, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return HP_OK;
+}
+
+int hp_set_donors(hp_ctx *c, int n, const int *recv, const int *donors, int ld, const double *weights)
+{
+    if (!c || n < 0 || (n > 0 && (!recv || !donors || !weights || ld < n)))
+        return fail(c, HP_ERR_ARG, "hp_set_donors: bad argument");
+    if (c->batch) return fail(c, HP_ERR_ARG, "hp_set_donors: the case belongs to a batch");
+    const int ne = c->m.ne;
+    std::vector<int> hr(n);
+    std::vector<int4> hid(n);
+    std::vector<double2> hw(2 * (size_t)n);
+    std::vector<char> is_recv(ne, 0);
+    for (int r = 0; r < n; r++) {
+        if (recv[r] < 1 || recv[r] > ne) return fail(c, HP_ERR_ARG, "hp_set_donors: receptor id out of range");
+        if (is_recv[recv[r] - 1]) return fail(c, HP_ERR_ARG, "hp_set_donors: receptor listed twice");
+        is_recv[recv[r] - 1] = 1;
+        hr[r] = recv[r] - 1;
+    }
+    for (int r = 0; r < n; r++) {
+        int d[4];
+        for (int k = 0; k < 4; k++) {
+            d[k] = donors[(size_t)k * ld + r] - 1;
+            if (d[k] < 0 || d[k] >= ne) return fail(c, HP_ERR_ARG, "hp_set_donors: donor id out of range");
+            if (is_recv[d[k]]) return fail(c, HP_ERR_ARG, "hp_set_donors: a donor is itself a receptor");
+        }
+        hid[r] = make_int4(d[0], d[1], d[2], d[3]);
+        hw[2 * (size_t)r] = make_double2(weights[r], weights[(size_t)ld + r]);
+        hw[2 * (size_t)r + 1] = make_double2(weights[(size_t)2 * ld + r], weights[(size_t)3 * ld + r]);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (void *p : c->dbufs) HIPCHK(c, hipFree(p));
+    c->dbufs.clear();
+    c->D = Donors{};
+    if (n == 0) return HP_OK;
+    void *pr = nullptr, *pi = nullptr, *pw = nullptr;
+    HIPCHK(c, hipMalloc(&pr, sizeof(int) * n));
+    c->dbufs.push_back(pr);
+    HIPCHK(c, hipMalloc(&pi, sizeof(int4) * n));
+    c->dbufs.push_back(pi);
+    HIPCHK(c, hipMalloc(&pw, sizeof(double2) * 2 * (size_t)n));
+    c->dbufs.push_back(pw);
+    HIPCHK(c, hipMemcpyAsync(pr, hr.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(pi, hid.data(), sizeof(int4) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(pw, hw.data(), sizeof(double2) * 2 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->D.n = n;
+    c->D.recv = static_cast<const int *>(pr);
+    c->D.ids = static_cast<const int4 *>(pi);
+    c->D.w = static_cast<const double2 *>(pw);
+    return HP_OK;
+}
+
+int hp_apply_donors(hp_ctx *c)
+{
+    if (!c) return fail(nullptr, HP_ERR_ARG, "null context");
+    launch_donor_gather(c->s, c->D, c->st);
+    return finish(c, "apply_donors");
 }
 
 static int sync_owner(hp_ctx *c)
@@ -781,6 +845,7 @@ int hp_batch_destroy(hp_batch *b)
 int hp_batch_add(hp_batch *b, hp_ctx *c, int *slot)
 {
     if (!b || !c) return bfail(b, HP_ERR_ARG, "hp_batch_add: null argument");
+    if (c->D.n) return bfail(b, HP_ERR_ARG, "hp_batch_add: a case with overset donor lists runs on its own");
     if (c->batch) return bfail(b, HP_ERR_ARG, "hp_batch_add: the case already belongs to a batch");
     if (b->ncases > 0 && (c->P.precision != b->G.precision || c->P.myflux != b->G.myflux ||
                           c->P.myorder != b->G.myorder))

@@ -105,6 +105,15 @@ void launch_rk_update(const Batch &B, const Grid &G, int stage, hipStream_t st);
 void launch_end_step(const Batch &B, const Grid &G, int want_diffs, hipStream_t st);
 void launch_dt_from_prims(const Batch &B, const Grid &G, hipStream_t st);
 
+// overset donor lists of one case: per receptor four donor ids and weights
+struct Donors {
+    int n;
+    const int *recv;      // n
+    const int4 *ids;      // n, 0-based
+    const double2 *w;     // 2n: (w1, w2), (w3, w4)
+};
+void launch_donor_gather(const State &s, const Donors &D, hipStream_t st);
+
 }  // namespace hydra
 
 #endif

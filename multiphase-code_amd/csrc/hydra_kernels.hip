@@ -1893,4 +1893,25 @@ void launch_end_step(const Batch &B, const Grid &G, int want_diffs, hipStream_t 
 
 #undef PICK
 
+// overset: each receptor's conserved variables from its four donors,
+// w1*q1 + w2*q2 + w3*q3 + w4*q4 in that order (receptors are never donors,
+// so the reads never see this launch's writes)
+__global__ void __launch_bounds__(256) k_donor_gather(State s, Donors D)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= D.n) return;
+    const int4 d = D.ids[r];
+    const double2 wa = D.w[2 * r], wb = D.w[2 * r + 1];
+    const int i = D.recv[r], ne = s.ne;
+    for (int k = 0; k < 10; k++) {
+        const double *c = s.cn + (size_t)k * ne;
+        s.cn[(size_t)k * ne + i] = wa.x * c[d.x] + wa.y * c[d.y] + wb.x * c[d.z] + wb.y * c[d.w];
+    }
+}
+
+void launch_donor_gather(const State &s, const Donors &D, hipStream_t st)
+{
+    if (D.n > 0) hipLaunchKernelGGL(k_donor_gather, dim3((D.n + 255) / 256), dim3(256), 0, st, s, D);
+}
+
 }  // namespace hydra

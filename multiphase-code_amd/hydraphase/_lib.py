@@ -44,7 +44,7 @@ EXPORTS = ["hp_create", "hp_destroy", "hp_upload_state", "hp_download_state", "h
            "hp_set_array", "hp_last_timing", "hp_probe_cell",
            "hp_batch_create", "hp_batch_destroy", "hp_batch_add", "hp_batch_remove", "hp_batch_set_grid",
            "hp_batch_step", "hp_batch_synchronize", "hp_batch_last_ms", "hp_batch_last_error",
-           "hp_case_bytes"]
+           "hp_case_bytes", "hp_set_donors", "hp_apply_donors"]
 
 
 def lib():
@@ -85,6 +85,8 @@ def lib():
     L.hp_batch_last_ms.argtypes = [vp, dp]
     L.hp_batch_last_error.argtypes = [vp]
     L.hp_batch_last_error.restype = ctypes.c_char_p
+    L.hp_set_donors.argtypes = [vp, c_int, ip, ip, c_int, dp]
+    L.hp_apply_donors.argtypes = [vp]
     L.hp_case_bytes.argtypes = [c_int, c_int, c_int, c_int]
     L.hp_case_bytes.restype = c_size
     for n in EXPORTS:

@@ -131,6 +131,57 @@ def box_mesh(nx: int, ny: int, nz: int, *, lo=(-0.5, -0.5, -0.5), hi=(0.5, 0.5, 
     return _connect(x, y, z, nod, lo, hi, (nx, ny, nz))
 
 
+def annulus_mesh(nr: int, ntheta: int, nz: int, *, r0: float = 0.1, r1: float = 0.25, center=(0.0, 0.0),
+                 zlo: float = -0.5, zhi: float = 0.5, box_lo=(-0.5, -0.5, -0.5), box_hi=(0.5, 0.5, 0.5)):
+    """Near-body mesh around a z-aligned cylinder: the shell r0 <= r <= r1
+    split into nr x ntheta x nz curved hexes (periodic in theta), each
+    Kuhn-split into six tets in (r, theta, z) index space, which maps
+    (r, theta, z) -> (x, y, z) with positive orientation.  Returns
+    (x, y, z, nod (ne, 4)) for combine(); the inner surface becomes a wall
+    (type 5), z faces type 53; the outer surface is left to the overset
+    fringe (type 5 ghosts on cells that are receptors)."""
+    i, j, k = np.meshgrid(np.arange(nr + 1), np.arange(ntheta), np.arange(nz + 1), indexing="ij")
+    i, j, k = i.transpose(2, 1, 0).ravel(), j.transpose(2, 1, 0).ravel(), k.transpose(2, 1, 0).ravel()
+    r = r0 + (r1 - r0) * i / nr
+    th = 2.0 * np.pi * j / ntheta
+    x = center[0] + r * np.cos(th)
+    y = center[1] + r * np.sin(th)
+    z = zlo + (zhi - zlo) * k / nz
+
+    def nid(a, b, c):
+        return a + (nr + 1) * ((b % ntheta) + ntheta * c)
+
+    ha, hb, hc = np.meshgrid(np.arange(nr), np.arange(ntheta), np.arange(nz), indexing="ij")
+    ha, hb, hc = ha.transpose(2, 1, 0).ravel(), hb.transpose(2, 1, 0).ravel(), hc.transpose(2, 1, 0).ravel()
+    corner = np.stack([nid(ha + (c & 1), hb + ((c >> 1) & 1), hc + ((c >> 2) & 1)) for c in range(8)], axis=1)
+    nod = corner[:, _kuhn_tets()].reshape(-1, 4)
+    return x, y, z, nod
+
+
+def combine(parts, lo=(-0.5, -0.5, -0.5), hi=(0.5, 0.5, 0.5), shape=(0, 0, 0)) -> Mesh:
+    """One mesh of several disconnected node/tet sets [(x, y, z, nod), ...]
+    (cells numbered part by part); faces and boundary types as box_mesh."""
+    xs, ys, zs, nods, off = [], [], [], [], 0
+    for (x, y, z, nod) in parts:
+        xs.append(x); ys.append(y); zs.append(z)
+        nods.append(np.asarray(nod, dtype=np.int64) + off)
+        off += x.size
+    return _connect(np.concatenate(xs), np.concatenate(ys), np.concatenate(zs), np.concatenate(nods),
+                    np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64), shape)
+
+
+def box_tets(nx: int, ny: int, nz: int, *, lo=(-0.5, -0.5, -0.5), hi=(0.5, 0.5, 0.5), jitter: float = 0.1,
+             seed: int = 1, keep=None):
+    """(x, y, z, nod (ne, 4)) of the Kuhn-split box of box_mesh; keep(cx, cy,
+    cz) -> bool mask selects the tets to keep by centroid."""
+    m = box_mesh(nx, ny, nz, lo=lo, hi=hi, jitter=jitter, seed=seed)
+    nod = m.nod.T.astype(np.int64)
+    if keep is not None:
+        c = [a[nod].mean(axis=1) for a in (m.x, m.y, m.z)]
+        nod = nod[keep(*c)]
+    return m.x, m.y, m.z, nod
+
+
 def _connect(x, y, z, nod, lo, hi, shape) -> Mesh:
     ne = nod.shape[0]
     fn = nod[:, FACE_NODES]  # (ne, 4, 3)

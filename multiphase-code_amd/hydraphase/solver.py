@@ -125,6 +125,20 @@ class Solver:
     def synchronize(self):
         self._chk(self.L.hp_synchronize(self.h))
 
+    # -- overset -----------------------------------------------------------
+    def set_donors(self, recv, donors, weights):
+        """Fixed donor lists (0-based receptor ids (n,), donors (n, 4),
+        weights (n, 4)); applied after every RK stage update."""
+        recv = np.ascontiguousarray(np.asarray(recv) + 1, dtype=np.int32)
+        n = recv.size
+        d = np.ascontiguousarray((np.asarray(donors) + 1).T, dtype=np.int32)   # (4, n) = column-major (n, 4)
+        w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).T)
+        self._donors = (recv, d, w)
+        self._chk(self.L.hp_set_donors(self.h, n, _ip(recv), _ip(d), n, _dp(w)))
+
+    def apply_donors(self):
+        self._chk(self.L.hp_apply_donors(self.h))
+
     def last_timing(self):
         t = ctypes.c_double()
         f = ctypes.c_double()

@@ -44,6 +44,10 @@ struct hyo_case {
     double *vis;                 /* 8 x neles */
     double dt, time;
     int total;
+    /* overset donor lists (no reference counterpart, see hyo_set_donors) */
+    int ndon;
+    int *drecv, *dids; /* ndon, 4 x ndon (row r: dids[4r..4r+3]) */
+    double *dw;        /* 4 x ndon */
 };
 
 #define CN(i, k) c->cn[(size_t)(k) * c->ntot + (i)]
@@ -279,6 +283,7 @@ void hyo_destroy(hyo_case *c)
     free(c->dd); free(c->dl); free(c->epssq); free(c->cn); free(c->cu);
     free(c->cn1); free(c->d2q); free(c->rhs); free(c->d24); free(c->anu);
     free(c->q); free(c->grad); free(c->phi); free(c->vis);
+    free(c->drecv); free(c->dids); free(c->dw);
     free(c);
 }
 
@@ -1437,6 +1442,36 @@ void hyo_rk_stage(hyo_case *c, int s)
                            (960.0 / 1331.0) * (CN(i, j) - (11.0 / 20.0) * dtbyvol * r);
             }
         }
+    }
+    hyo_donor_gather(c);  /* overset receptors (none unless hyo_set_donors) */
+}
+
+/* Overset donor gather.  The reference has none (it only freezes iblank
+ * cells); this restates the rule of the fixed donor lists the HIP library
+ * applies after each stage update: receptor r takes, per conserved
+ * variable, w1 cn(d1) + w2 cn(d2) + w3 cn(d3) + w4 cn(d4) in that order.
+ * Parity against the reference: none possible (no reference function). */
+void hyo_set_donors(hyo_case *c, int n, const int *recv, const int *ids, const double *w)
+{
+    free(c->drecv); free(c->dids); free(c->dw);
+    c->drecv = NULL; c->dids = NULL; c->dw = NULL;
+    c->ndon = n;
+    if (n <= 0) { c->ndon = 0; return; }
+    c->drecv = malloc(sizeof(int) * n);
+    c->dids = malloc(sizeof(int) * 4 * (size_t)n);
+    c->dw = malloc(sizeof(double) * 4 * (size_t)n);
+    memcpy(c->drecv, recv, sizeof(int) * n);
+    memcpy(c->dids, ids, sizeof(int) * 4 * (size_t)n);
+    memcpy(c->dw, w, sizeof(double) * 4 * (size_t)n);
+}
+
+void hyo_donor_gather(hyo_case *c)
+{
+    for (int r = 0; r < c->ndon; r++) {
+        const int *d = c->dids + 4 * (size_t)r;
+        const double *w = c->dw + 4 * (size_t)r;
+        for (int j = 0; j < NEQ; j++)
+            CN(c->drecv[r], j) = w[0] * CN(d[0], j) + w[1] * CN(d[1], j) + w[2] * CN(d[2], j) + w[3] * CN(d[3], j);
     }
 }
 

@@ -93,6 +93,11 @@ void hyo_source(hyo_case *c);           /* sr_source.F90 */
 void hyo_flux_solver(hyo_case *c);      /* mod_Fluxmodule.F90:7-21 */
 void hyo_rk_stage(hyo_case *c, int s);  /* sr_RK.F90 stage s (1..4) update */
 void hyo_rk(hyo_case *c);               /* sr_RK.F90 */
+/* overset donor lists (0-based; ids row-major n x 4, weights n x 4) and
+ * their gather; hyo_rk_stage applies the lists after its update.  No
+ * reference counterpart. */
+void hyo_set_donors(hyo_case *c, int n, const int *recv, const int *ids, const double *w);
+void hyo_donor_gather(hyo_case *c);
 /* one time step, main.F90:515-642; diffs (4) = max|cn-cu| vars 2,3,4,1 */
 void hyo_step(hyo_case *c, double *diffs);
 void hyo_run(hyo_case *c, int nsteps);

@@ -77,6 +77,8 @@ def lib():
         L.hyo_array.argtypes = [vp, ctypes.c_int]
         L.hyo_array.restype = ctypes.c_void_p
         L.hyo_get_dt.argtypes = [vp]
+        L.hyo_set_donors.argtypes = [vp, ctypes.c_int, ip, ip, dp]
+        L.hyo_donor_gather.argtypes = [vp]
         L.hyo_get_dt.restype = ctypes.c_double
         _lib = L
     return _lib
@@ -162,3 +164,11 @@ class Oracle:
 
     def run(self, n):
         lib().hyo_run(self.h, int(n))
+
+    def set_donors(self, recv, donors, weights):
+        """Overset donor lists, 0-based (see hyo_set_donors)."""
+        self._don = (np.ascontiguousarray(recv, dtype=np.int32), np.ascontiguousarray(donors, dtype=np.int32),
+                     np.ascontiguousarray(weights, dtype=np.float64))
+        r, d, w = self._don
+        lib().hyo_set_donors(self.h, int(r.size), r.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                             d.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(w))
