@@ -1,18 +1,22 @@
 """Throughput of the HydraPhase per-step FV update on one MI355X.
 
-Workload (the single large case): a 96 x 96 x 90 hex box (unit cross
-section, x in [-0.5,0.5], y in [-0.5,0.5], z in [-0.5,0.5]) Kuhn-split into
-4,976,640 tets with interior nodes jittered by +-10% of the spacing (seed 1),
-the tets inside a z-cylinder of radius 0.1 removed and their faces turned
-into wall ghosts (type 5), gas viscosity on (air 1.8e-5 Pa s via
-Sutherland, water 1e-3 Pa s), a train of 8 water spheres (radii 0.03-0.08,
-seeded) in an air stream at Mach 0.3, CFL 0.5.  FP64 throughout.
+Workload (the single large case, two overlapping meshes): a 96 x 96 x 90
+hex box (x, y, z in [-0.5, 0.5]) Kuhn-split into tets with interior nodes
+jittered by +-10% of the spacing (seed 1), its tets of centroid radius
+< 0.15 from the z axis cut out, plus a near-body annulus around the
+cylinder r = 0.1 (no-slip wall, type 5 ghosts) out to r = 0.25, 10 x 112 x
+90 curved hexes x 6 = 604,800 tets; the two are coupled by fixed
+barycentric donor lists (two fringe layers each side, applied after every
+RK stage).  Gas viscosity on (air 1.8e-5 Pa s via Sutherland, water 1e-3
+Pa s), a train of 8 water spheres (radii 0.03-0.08, seeded) in an air
+stream at Mach 0.3, CFL 0.5.  FP64 throughout (--precision 1: FP32 bulk).
+--single-mesh runs the box alone with the cylinder as a wall hole.
 
 A step is one full time step of the reference (main.F90:515-625): primitive,
 CFL reduction, JST smoothing, viscous fluxes and four RK stages of
-primitiveplusbc + gradients + limiter + HLLC face flux + source + update.
-The metric counts cell-stage updates (active cells x 4 stages) per second
-with the state resident in HBM.
+primitiveplusbc + gradients + limiter + HLLC face flux + source + update
+(+ donor gather).  The metric counts cell-stage updates (active cells x 4
+stages) per second with the state resident in HBM.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
        python bench.py summarize-counters FETCH.csv WRITE.csv CALIB_MB
@@ -21,7 +25,7 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
 roofline.traffic is the HBM bytes of one k_flux launch measured with
 rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (each run with
 --calib-copy-mb 1024 and summarised by summarize-counters into
-profiles/k_flux_box96x96x90_fp64_counters.md); FETCH_SIZE and WRITE_SIZE
+profiles/k_flux_overset96x96x90_fp64_counters.md); FETCH_SIZE and WRITE_SIZE
 are scaled by what they read for a float64 b=2a kernel of known size.
 """
 from __future__ import annotations
@@ -49,18 +53,27 @@ FLUX_BYTES_PER_CELL = 112 + 264 + 8 + 96 + 96 + 24 + 16 + 16 + 64 + 80 + 80 + 60
 FLUX_BYTES_PER_CELL_F32 = FLUX_BYTES_PER_CELL - 132 - 4
 
 
-def build_case(nx, ny, nz, seed=1, precision=0):
+def build_case(nx, ny, nz, seed=1, precision=0, overset=True):
+    """(mesh, params, cu, gp, donors) of the single large case; donors is
+    (recv, ids, weights) or None for the single-mesh form."""
     from hydraphase import mesh as M
+    from hydraphase import overset as OV
     hole = (0.0, 0.0, 0.1)
-    m = M.box_mesh(nx, ny, nz, jitter=0.1, seed=seed, hole=hole)
+    donors = None
+    if overset:
+        m, recv, ids, w = OV.overset_mesh(nx, ny, nz, nr=10, ntheta=112, r_body=0.1, r_outer=0.25,
+                                          r_cut=0.15, fringe=2, jitter=0.1, seed=seed)
+        donors = (recv, ids, w)
+    else:
+        m = M.box_mesh(nx, ny, nz, jitter=0.1, seed=seed, hole=hole)
     p = M.water_air_params(cfl=0.5, viscous_gas=True)
     p["precision"] = precision
-    spheres = M.sphere_train(seed, n=8, rmin=0.03, rmax=0.08, avoid=hole)
+    spheres = M.sphere_train(seed, n=8, rmin=0.03, rmax=0.08, avoid=(0.0, 0.0, 0.25 if overset else 0.1))
     cu, gp = M.sphere_state(m, p, spheres)
-    return m, p, cu, gp
+    return m, p, cu, gp, donors
 
 
-def cpu_baseline(m, p, cu, gp, seconds_target=20.0):
+def cpu_baseline(m, p, cu, gp, donors=None, seconds_target=20.0):
     """The serial C oracle (a port of the reference's FP64 path, 1 thread)
     on a bounded sample of the same workload: whole steps of the same case
     until about seconds_target of CPU time."""
@@ -68,6 +81,9 @@ def cpu_baseline(m, p, cu, gp, seconds_target=20.0):
     import oracle as O
     o = O.Oracle(m, p)
     o.set_state(cu, gp, 0.0, 0)
+    if donors is not None:
+        o.set_donors(*donors)
+        o.call("donor_gather")
     nsteps, t0 = 0, time.perf_counter()
     while True:
         o.step()
@@ -83,7 +99,7 @@ def cpu_baseline(m, p, cu, gp, seconds_target=20.0):
 
 
 PROFILES = os.path.join(ROOT, "profiles")
-TRAFFIC_FILE = os.path.join(PROFILES, "k_flux_box96x96x90_fp64_counters.md")
+TRAFFIC_FILE = os.path.join(PROFILES, "k_flux_overset96x96x90_fp64_counters.md")
 
 
 def flux_traffic_bytes():
@@ -240,6 +256,8 @@ def main():
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1],
                     help="0: FP64 drop-in (the headline); 1: FP32-bulk variant")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--single-mesh", action="store_true",
+                    help="the box alone with the cylinder as a wall hole (no annulus, no donors)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched 64-case ensemble")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop run")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -264,12 +282,16 @@ def main():
         torch.cuda.synchronize()
         del a, b
     t0 = time.time()
-    m, p, cu, gp = build_case(*args.n, precision=args.precision)
-    print(f"# case built: {m.neles} tets, {m.nghosts} ghosts ({time.time() - t0:.1f} s)", file=sys.stderr,
+    m, p, cu, gp, donors = build_case(*args.n, precision=args.precision, overset=not args.single_mesh)
+    print(f"# case built: {m.neles} tets, {m.nghosts} ghosts, "
+          f"{0 if donors is None else donors[0].size} receptors ({time.time() - t0:.1f} s)", file=sys.stderr,
           flush=True)
     from hydraphase import Solver
     s = Solver(m, p)
     s.upload(cu, gp, 0.0, 0)
+    if donors is not None:
+        s.set_donors(*donors)
+        s.apply_donors()
     if args.warmup:
         s.step(args.warmup)
         s.synchronize()
@@ -300,13 +322,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == 0 else "f32-bulk/f64-state",
         "data": "synthetic",
-        "config": {"workload": f"box {args.n[0]}x{args.n[1]}x{args.n[2]} hexes x6 tets, cylinder wall "
-                               f"r=0.1, 8 water spheres, viscous, "
-                               + ("FP64" if args.precision == 0 else "FP32 bulk"),
-                   "tets": m.neles, "active_cells": active, "ghosts": m.nghosts, "stages_per_step": 4},
+        "config": {"workload": (f"box {args.n[0]}x{args.n[1]}x{args.n[2]} hexes x6 tets cut at r=0.15 + "
+                                "annulus 10x112 hexes x6 around the r=0.1 wall, overset donor lists"
+                                if donors is not None else
+                                f"box {args.n[0]}x{args.n[1]}x{args.n[2]} hexes x6 tets, cylinder wall hole r=0.1")
+                               + ", 8 water spheres, viscous, " + ("FP64" if args.precision == 0 else "FP32 bulk"),
+                   "tets": m.neles, "active_cells": active, "ghosts": m.nghosts,
+                   "receptors": 0 if donors is None else int(donors[0].size), "stages_per_step": 4},
         "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": flux_traffic_bytes() if list(args.n) == [96, 96, 90] and args.precision == 0 else None,
+                     "traffic": (flux_traffic_bytes() if list(args.n) == [96, 96, 90] and args.precision == 0
+                                 and not args.single_mesh else None),
                      "flux_avg_ms": flux_avg_ms, "bytes_per_cell": bpc},
     }
     s.close()
@@ -318,7 +344,7 @@ def main():
         out["serve"] = serve_bench(log)
         log(f"# serve: p50 {out['serve']['tts_p50_s']:.2f} s p99 {out['serve']['tts_p99_s']:.2f} s")
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(m, p, cu, gp, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(m, p, cu, gp, donors, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -93,6 +93,18 @@ def box_mesh(nx: int, ny: int, nz: int, *, lo=(-0.5, -0.5, -0.5), hi=(0.5, 0.5, 
     +-jitter*h per axis (seeded).  ``hole=(xc, yc, r)`` removes the tets whose
     centroid lies inside the z-aligned cylinder; the faces they shared become
     wall ghosts (type 5)."""
+    x, y, z, nod = _box_nodes_tets(nx, ny, nz, lo, hi, jitter, seed)
+    if hole is not None:
+        xc, yc, r = hole
+        cx = x[nod].mean(axis=1)
+        cy = y[nod].mean(axis=1)
+        keep = (cx - xc) ** 2 + (cy - yc) ** 2 > r * r
+        nod = nod[keep]
+    return _connect(x, y, z, nod, np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64),
+                    (nx, ny, nz))
+
+
+def _box_nodes_tets(nx, ny, nz, lo, hi, jitter, seed):
     lo = np.asarray(lo, dtype=np.float64)
     hi = np.asarray(hi, dtype=np.float64)
     h = (hi - lo) / np.array([nx, ny, nz], dtype=np.float64)
@@ -122,13 +134,7 @@ def box_mesh(nx: int, ny: int, nz: int, *, lo=(-0.5, -0.5, -0.5), hi=(0.5, 0.5, 
                              for c in range(8)], axis=1)  # (nhex, 8)
     kt = _kuhn_tets()
     nod = corner_nodes[:, kt].reshape(-1, 4)  # (neles, 4), hex-major
-    if hole is not None:
-        xc, yc, r = hole
-        cx = x[nod].mean(axis=1)
-        cy = y[nod].mean(axis=1)
-        keep = (cx - xc) ** 2 + (cy - yc) ** 2 > r * r
-        nod = nod[keep]
-    return _connect(x, y, z, nod, lo, hi, (nx, ny, nz))
+    return x, y, z, nod
 
 
 def annulus_mesh(nr: int, ntheta: int, nz: int, *, r0: float = 0.1, r1: float = 0.25, center=(0.0, 0.0),
@@ -174,12 +180,11 @@ def box_tets(nx: int, ny: int, nz: int, *, lo=(-0.5, -0.5, -0.5), hi=(0.5, 0.5, 
              seed: int = 1, keep=None):
     """(x, y, z, nod (ne, 4)) of the Kuhn-split box of box_mesh; keep(cx, cy,
     cz) -> bool mask selects the tets to keep by centroid."""
-    m = box_mesh(nx, ny, nz, lo=lo, hi=hi, jitter=jitter, seed=seed)
-    nod = m.nod.T.astype(np.int64)
+    x, y, z, nod = _box_nodes_tets(nx, ny, nz, lo, hi, jitter, seed)
     if keep is not None:
-        c = [a[nod].mean(axis=1) for a in (m.x, m.y, m.z)]
+        c = [a[nod].mean(axis=1) for a in (x, y, z)]
         nod = nod[keep(*c)]
-    return m.x, m.y, m.z, nod
+    return x, y, z, nod
 
 
 def _connect(x, y, z, nod, lo, hi, shape) -> Mesh:

@@ -38,13 +38,15 @@ def centroids(m: M.Mesh):
 
 def barycentric(p, a, b, c, d):
     """Barycentric coordinates of points p (n,3) in tets (a,b,c,d) (n,3)
-    each; returns (n, 4) and the tet volumes x6 (n,)."""
-    m = np.stack([b - a, c - a, d - a], axis=2)  # (n, 3, 3) columns
-    det = np.linalg.det(m)
-    ok = np.abs(det) > 1e-300
-    lam = np.zeros((p.shape[0], 3))
-    lam[ok] = np.linalg.solve(m[ok], (p - a)[ok][..., None])[..., 0]
-    w = np.concatenate([1.0 - lam.sum(axis=1, keepdims=True), lam], axis=1)
+    each, by Cramer's rule; returns (n, 4) and the tet volumes x6 (n,)."""
+    v0, v1, v2, dp = b - a, c - a, d - a, p - a
+    c12 = np.cross(v1, v2)
+    det = np.einsum("ij,ij->i", v0, c12)
+    inv = np.divide(1.0, det, out=np.zeros_like(det), where=np.abs(det) > 1e-300)
+    l1 = np.einsum("ij,ij->i", dp, c12) * inv
+    l2 = np.einsum("ij,ij->i", v0, np.cross(dp, v2)) * inv
+    l3 = np.einsum("ij,ij->i", v0, np.cross(v1, dp)) * inv
+    w = np.stack([1.0 - l1 - l2 - l3, l1, l2, l3], axis=1)
     return w, det
 
 
