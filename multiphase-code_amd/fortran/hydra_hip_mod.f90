@@ -105,6 +105,17 @@ module hydra_hip_mod
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function hp_set_donors(ctx, n, recv, donors, ld, weights) bind(C, name="hp_set_donors")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n, ld
+      integer(c_int), intent(in) :: recv(*), donors(*)
+      real(c_double), intent(in) :: weights(*)
+    end function
+    integer(c_int) function hp_apply_donors(ctx) bind(C, name="hp_apply_donors")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
     type(c_ptr) function hp_last_error_c(ctx) bind(C, name="hp_last_error")
       import :: c_ptr
       type(c_ptr), value :: ctx
