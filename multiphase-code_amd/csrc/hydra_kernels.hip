@@ -934,9 +934,9 @@ __device__ __forceinline__ void hllc_zien(const Params &P, const FState &L, cons
 // stage >= 1: apply the RK stage update (sr_RK.F90) from the fused residual;
 // stage == 0: store the residual in rhs (per-routine checks).
 // Interphase source (sr_source.F90:46-155) and the RK stage update
-// (sr_RK.F90:29-161) of the quad's cell i from its face-summed residual rhs
+// (sr_RK.F90:29-161) of the quad's cell i from its face-summed residual
 // (FP64 in every precision).  Lane f of the quad finishes the conserved
-// variables k = f, f+4, f+8.  stage >= 1: update cn (and cn1); stage == 0:
+// variables k = f, f+4, f+8, whose face sums are rhs[0], rhs[1], rhs[2].  stage >= 1: update cn (and cn1); stage == 0:
 // store the residual in rhs (per-routine checks).
 __device__ __forceinline__ void source_and_update(const Params &P, const State &s, const Mesh &m, int i, int f,
                                                   int qbase, int neigh, const double *rhs, int stage)
@@ -984,7 +984,7 @@ __device__ __forceinline__ void source_and_update(const Params &P, const State &
                         wint * fdgz + qsource;
     const double dtbyvol = (stage > 0) ? s.scal[SC_DT] / voli : 0.0;
     for (int k = f; k < 10; k += 4) {
-        double r = rhs[k];
+        double r = rhs[k >> 2];
         const double *vis = s.vis;
         switch (k) {
         case 1: r = r - (fdgx + pint * dphx + phig * rog * P.gravity) * voli - vis[(size_t)0 * ne + i]; break;
@@ -1082,11 +1082,13 @@ __device__ __forceinline__ void flux_face(const Params &P, const State &s, const
                               R.wp, R.tp, R.pp, enx, eny, enz, area, F + 5);
         }
     }
-    double rhs[10];
+    // lane f keeps the face sums of its own variables k = f, f+4, f+8
+    double rhs[3] = {0.0, 0.0, 0.0};
     for (int k = 0; k < 10; k++) {
         const double t0 = quad_get(F[k], qbase, 0), t1 = quad_get(F[k], qbase, 1),
                      t2 = quad_get(F[k], qbase, 2), t3 = quad_get(F[k], qbase, 3);
-        rhs[k] = 0.0 + t0 + t1 + t2 + t3;
+        const double sum = 0.0 + t0 + t1 + t2 + t3;
+        if ((k & 3) == f) rhs[k >> 2] = sum;
     }
     source_and_update(P, s, m, i, f, qbase, neigh, rhs, stage);
 }
@@ -1535,11 +1537,12 @@ __device__ __forceinline__ void flux_face_f32(const Params &P, const State &s, c
                                 R.wp, R.tp, R.pp, enx, eny, enz, area, F + 5);
         }
     }
-    double rhs[10];
+    double rhs[3] = {0.0, 0.0, 0.0};
     for (int k = 0; k < 10; k++) {
         const double t0 = (double)__shfl(F[k], qbase, 64), t1 = (double)__shfl(F[k], qbase + 1, 64),
                      t2 = (double)__shfl(F[k], qbase + 2, 64), t3 = (double)__shfl(F[k], qbase + 3, 64);
-        rhs[k] = 0.0 + t0 + t1 + t2 + t3;
+        const double sum = 0.0 + t0 + t1 + t2 + t3;
+        if ((k & 3) == f) rhs[k >> 2] = sum;
     }
     source_and_update(P, s, m, i, f, qbase, neigh, rhs, stage);
 }
