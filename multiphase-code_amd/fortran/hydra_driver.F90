@@ -5,7 +5,8 @@
 ! (main.F90:511-657) with every routine executed by libhydra_hip.so through
 ! hydra_hip_mod.  Per step it prints the reference's monitor line
 ! (main.F90:653) and appends time and |u_g| of cell 1 to Velocity.dat
-! (:641,654); at the end it writes restart_files/restart0000.in (:812-828).
+! (:641,654); at the end it writes restart_files/restart0000.in (:812-828)
+! and tecplot_files/tecplot0000.dat (:956-1266).
 !
 ! HYDRA_FUSED=1 in the environment runs each outer iteration as one
 ! device-resident hp_step call (no per-step host synchronisation, no
@@ -145,8 +146,82 @@ program hydra_driver
   open(newunit=u, file='itter.in', status='replace')
   write(u, *) itter
   close(u)
+  call write_tecplot()
 
 contains
+
+  ! main.F90:956-1266: Tecplot FE file of the final state, primitives from
+  ! cu by the reference's two-phase recovery; the reference names 16
+  ! variables and then writes 17 blocks (PHIP last), reproduced as is
+  subroutine write_tecplot()
+    real(dp), allocatable :: w(:, :)
+    real(dp) :: eg, ep, bbb, ccc, psig, fg, rrg, s4
+    integer :: ie, uo, j
+    allocate(w(neles, 14))
+    rrg = 287.0_dp
+    s4 = 4.0_dp * prm%surfacetension / prm%diap
+    do ie = 1, neles
+      associate (ug => w(ie, 1), vg => w(ie, 2), wg => w(ie, 3), pg => w(ie, 4), pp => w(ie, 5), &
+                 tg => w(ie, 6), up => w(ie, 7), vp => w(ie, 8), wp => w(ie, 9), phig => w(ie, 10), &
+                 tp => w(ie, 11), rog => w(ie, 12), rop => w(ie, 13), phip => w(ie, 14))
+        ug = cu(ie, 2) / cu(ie, 1); vg = cu(ie, 3) / cu(ie, 1); wg = cu(ie, 4) / cu(ie, 1)
+        up = cu(ie, 7) / cu(ie, 6); vp = cu(ie, 8) / cu(ie, 6); wp = cu(ie, 9) / cu(ie, 6)
+        eg = cu(ie, 5) / cu(ie, 1) - 0.5_dp * (ug**2 + vg**2 + wg**2)
+        ep = cu(ie, 10) / cu(ie, 6) - 0.5_dp * (up**2 + vp**2 + wp**2)
+        bbb = (s4 + prm%pinf * prm%gamap) - (cu(ie, 1) * (prm%gamag - 1.0_dp) * eg) - &
+              (cu(ie, 6) * (prm%gamap - 1.0_dp) * ep)
+        ccc = (cu(ie, 1) * (prm%gamag - 1.0_dp) * eg * (s4 + prm%pinf * prm%gamap))
+        pg = (-bbb + sqrt(bbb**2 + 4 * ccc)) * 0.5_dp
+        pp = pg + s4
+        phig = (cu(ie, 1) * (prm%gamag - 1.0_dp) * eg) / pg
+        phip = 1.0_dp - phig
+        rog = cu(ie, 1) / phig
+        rop = cu(ie, 6) / phip
+        tg = pg / (rog * rrg)
+        tp = ((pp + prm%pinf) * prm%gamap) / ((prm%gamap - 1.0_dp) * rop * prm%cpp)
+        if (phig < prm%epslnmin) then
+          phig = prm%epslnmin; phip = 1.0_dp - phig
+          ug = up; vg = vp; wg = wp; tg = tp; pg = pp
+        end if
+        if (phip < prm%epslnmin) then
+          phip = prm%epslnmin; phig = 1.0_dp - phip
+          up = ug; vp = vg; wp = wg; tp = tg; pp = pg
+        end if
+        if (phig >= prm%epslnmin .and. phig <= prm%epslnmax) then
+          psig = (phig - prm%epslnmin) / (prm%epslnmax - prm%epslnmin)
+          fg = -psig * psig * (2.0_dp * psig - 3.0_dp)
+          ug = fg * ug + (1.0_dp - fg) * up; vg = fg * vg + (1.0_dp - fg) * vp
+          wg = fg * wg + (1.0_dp - fg) * wp; tg = fg * tg + (1.0_dp - fg) * tp
+          pg = fg * pg + (1.0_dp - fg) * pp
+        end if
+        if (phip >= prm%epslnmin .and. phip <= prm%epslnmax) then
+          psig = (phip - prm%epslnmin) / (prm%epslnmax - prm%epslnmin)
+          fg = -psig * psig * (2.0_dp * psig - 3.0_dp)
+          up = fg * up + (1.0_dp - fg) * ug; vp = fg * vp + (1.0_dp - fg) * vg
+          wp = fg * wp + (1.0_dp - fg) * wg; tp = fg * tp + (1.0_dp - fg) * tg
+          pp = fg * pp + (1.0_dp - fg) * pg
+        end if
+        rog = pg / (rrg * tg)
+        rop = (pp + prm%pinf) * prm%gamap / ((prm%gamap - 1.0_dp) * prm%cpp * tp)
+      end associate
+    end do
+    call execute_command_line('mkdir -p tecplot_files')
+    open(newunit=uo, file='tecplot_files/tecplot0000.dat', status='replace', action='write')
+    write(uo, '(A)') 'VARIABLES = "X", "Y", "Z", "U","V", "W", "PG", "PP", "TG", ' // &
+                     '"UP", "VP", "WP", "PHIG", "TP", "ROG", "ROP"'
+    write(uo, '(A,I0,A,I0,A,I0,A)') 'ZONE ZONETYPE=FETETRAHEDRON, NODES=', nodes, ', ELEMENTS=', neles, &
+        ', DATAPACKING=BLOCK, VARLOCATION=([4-', 16, ']=CELLCENTERED)'
+    write(uo, '(ES16.8)') x
+    write(uo, '(ES16.8)') y
+    write(uo, '(ES16.8)') z
+    do j = 1, 14
+      write(uo, '(ES16.8)') w(:, j)
+    end do
+    do ie = 1, neles
+      write(uo, *) nod(ie, 1), nod(ie, 2), nod(ie, 3), nod(ie, 4)
+    end do
+    close(uo)
+  end subroutine write_tecplot
 
   ! sr_Read_flow.F90:19-268: each tuple is the next line that is neither
   ! blank nor a '#' comment and parses, with any trailing comment removed

@@ -151,3 +151,82 @@ def write_case(dirname: str, mesh: Mesh, p: dict, cu, gp, time: float = 0.0, itt
     write_restart(os.path.join(dirname, "restart_files", "restart0000.in"), time, cu, gp)
     with open(os.path.join(dirname, "itter.in"), "w") as fh:
         fh.write(f"{itter}\n")
+
+
+TECPLOT_VARIABLES = ('VARIABLES = "X", "Y", "Z", "U","V", "W", "PG", "PP", "TG", '
+                     '"UP", "VP", "WP", "PHIG", "TP", "ROG", "ROP"')
+
+
+def tecplot_fields(p: dict, cu: np.ndarray) -> list:
+    """The 14 cell fields of the reference's Tecplot file, in its order
+    (U V W PG PP TG UP VP WP PHIG TP ROG ROP PHIP), recovered from cu by its
+    two-phase relations with clipping and cubic blending
+    (main.F90:1001-1183)."""
+    from .mesh import derive
+    d = derive(p)
+    gg, gp_, pinf, cpp = p["gamag"], p["gamap"], p["pinf"], p["cpp"]
+    emin, emax, rrg, s4 = p["epslnmin"], p["epslnmax"], d["rrg"], d["foursigmabyd"]
+    ug, vg, wg = cu[1] / cu[0], cu[2] / cu[0], cu[3] / cu[0]
+    up, vp, wp = cu[6] / cu[5], cu[7] / cu[5], cu[8] / cu[5]
+    eg = cu[4] / cu[0] - 0.5 * (ug ** 2 + vg ** 2 + wg ** 2)
+    ep = cu[9] / cu[5] - 0.5 * (up ** 2 + vp ** 2 + wp ** 2)
+    bbb = (s4 + pinf * gp_) - (cu[0] * (gg - 1.0) * eg) - (cu[5] * (gp_ - 1.0) * ep)
+    ccc = cu[0] * (gg - 1.0) * eg * (s4 + pinf * gp_)
+    pg = (-bbb + np.sqrt(bbb ** 2 + 4 * ccc)) * 0.5
+    pp = pg + s4
+    phig = (cu[0] * (gg - 1.0) * eg) / pg
+    phip = 1.0 - phig
+    rog, rop = cu[0] / phig, cu[5] / phip
+    tg = pg / (rog * rrg)
+    tp = ((pp + pinf) * gp_) / ((gp_ - 1.0) * rop * cpp)
+    a = phig < emin
+    phig = np.where(a, emin, phig); phip = np.where(a, 1.0 - phig, phip)
+    ug, vg, wg, tg, pg = (np.where(a, b, c) for b, c in ((up, ug), (vp, vg), (wp, wg), (tp, tg), (pp, pg)))
+    b_ = phip < emin
+    phip = np.where(b_, emin, phip); phig = np.where(b_, 1.0 - phip, phig)
+    up, vp, wp, tp, pp = (np.where(b_, b, c) for b, c in ((ug, up), (vg, vp), (wg, wp), (tg, tp), (pg, pp)))
+    band = (phig >= emin) & (phig <= emax)
+    psig = (phig - emin) / (emax - emin)
+    fg = -psig * psig * (2.0 * psig - 3.0)
+    ug, vg, wg, tg, pg = (np.where(band, fg * x + (1.0 - fg) * y, x)
+                          for x, y in ((ug, up), (vg, vp), (wg, wp), (tg, tp), (pg, pp)))
+    band = (phip >= emin) & (phip <= emax)
+    psig = (phip - emin) / (emax - emin)
+    fg = -psig * psig * (2.0 * psig - 3.0)
+    up, vp, wp, tp, pp = (np.where(band, fg * x + (1.0 - fg) * y, x)
+                          for x, y in ((up, ug), (vp, vg), (wp, wg), (tp, tg), (pp, pg)))
+    rog = pg / (rrg * tg)
+    rop = (pp + pinf) * gp_ / ((gp_ - 1.0) * cpp * tp)
+    return [ug, vg, wg, pg, pp, tg, up, vp, wp, phig, tp, rog, rop, phip]
+
+
+def write_tecplot(path: str, mesh: Mesh, p: dict, cu: np.ndarray) -> None:
+    """tecplot%04d.dat as the reference writes it (main.F90:956-1266): an FE
+    tetrahedral zone in block packing, node coordinates then 14 cell-centred
+    fields, one ES16.8 value per line, then the 1-based connectivity.  The
+    reference's header names 16 variables while 17 blocks follow (PHIP
+    last); that layout is kept so tools written for its files read ours."""
+    fields = tecplot_fields(p, cu)
+    with open(path, "w") as fh:
+        fh.write(TECPLOT_VARIABLES + "\n")
+        fh.write(f"ZONE ZONETYPE=FETETRAHEDRON, NODES={mesh.nodes}, ELEMENTS={mesh.neles}, "
+                 "DATAPACKING=BLOCK, VARLOCATION=([4-16]=CELLCENTERED)\n")
+        for arr in [mesh.x, mesh.y, mesh.z] + fields:
+            fh.write("".join(f"{v:16.8E}\n" for v in arr))
+        for e in range(mesh.neles):
+            n = mesh.nod[:, e] + 1
+            fh.write(f" {n[0]} {n[1]} {n[2]} {n[3]}\n")
+
+
+def read_tecplot(path: str):
+    """(nodes, neles, blocks (17, ...), connectivity (neles, 4) 1-based)."""
+    import re
+    with open(path) as fh:
+        fh.readline()
+        zone = fh.readline()
+        nodes = int(re.search(r"NODES=(\d+)", zone).group(1))
+        neles = int(re.search(r"ELEMENTS=(\d+)", zone).group(1))
+        blocks = [np.array([float(fh.readline()) for _ in range(nodes)]) for _ in range(3)]
+        blocks += [np.array([float(fh.readline()) for _ in range(neles)]) for _ in range(14)]
+        conn = np.array([[int(v) for v in fh.readline().split()] for _ in range(neles)])
+    return nodes, neles, blocks, conn

@@ -2,7 +2,8 @@
 libhydra_hip.so) on a run directory in the reference's own file formats:
 its restart file after 2 x 3 steps equals the oracle's 6-step state bit for
 bit, in both the per-routine (monitor output) and the device-resident
-fused mode."""
+fused mode, and its Tecplot file holds the same fields as the Python
+writer's."""
 import os
 import subprocess
 
@@ -42,3 +43,10 @@ def test_driver_restart_matches_oracle(tmp_path, fused):
     assert t == t_o
     np.testing.assert_array_equal(cu_d, cu_o)
     np.testing.assert_array_equal(gp_d, gp_o)
+    # its Tecplot file carries the same fields as the Python writer's
+    # (main.F90:956-1266 layout; 9 significant digits)
+    nodes, neles, blocks, conn = refio.read_tecplot(str(tmp_path / "tecplot_files" / "tecplot0000.dat"))
+    assert (nodes, neles) == (m.nodes, m.neles) and np.array_equal(conn, m.nod.T + 1)
+    want = refio.tecplot_fields(p, cu_o)
+    for got, ref in zip(blocks[3:], want):
+        np.testing.assert_allclose(got, ref, rtol=2e-8, atol=1e-300)
