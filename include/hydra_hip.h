@@ -57,9 +57,10 @@ typedef struct {
 /* precision (hp_params.precision): 0 = FP64 throughout (reference
  * arithmetic, bit-identical to the reference's serial evaluation);
  * 1 = FP32 bulk: gradients, limiter, face reconstruction, HLLC and viscous
- * face fluxes in FP32, with the conserved state, residual accumulation,
- * source, EOS inversion, boundary conditions, smoothing and dt reduction in
- * FP64.  Any other value is rejected by hp_create (HP_ERR_ARG). */
+ * face fluxes and interphase source terms in FP32, with the conserved
+ * state, residual accumulation, RK update, EOS inversion, boundary
+ * conditions, smoothing and dt reduction in FP64.  Any other value is
+ * rejected by hp_create (HP_ERR_ARG). */
 
 typedef struct hp_ctx hp_ctx;
 
