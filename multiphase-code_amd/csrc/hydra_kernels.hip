@@ -502,6 +502,86 @@ __device__ __forceinline__ void smooth2_cell(const Params &P, const State &s, co
 // plus the cell (boundary face) or plus the neighbour's 4 neighbours
 // (interior face); Newtonian stress and Fourier heat flux of both phases.
 // ---------------------------------------------------------------------------
+// Face-centred LSQ gradients of the eight face variables over the NM-cell
+// stencil mm (NM = 8 at an interior face: both cells' neighbours; NM = 5 at
+// a boundary face: the cell's neighbours and the cell) and the two phases'
+// stress and heat-flux contributions to the face (sr_viscousgp.F90:450-804).
+// NM is a constant so every stencil loop unrolls into straight-line code.
+template <int NM>
+__device__ __forceinline__ void viscous_stencil(const Params &P, const double *q, size_t nt, const Mesh &m,
+                                                const int (&mm)[8], double xf, double yf, double zf,
+                                                double scx, double scy, double scz, const double (&fv)[8],
+                                                double phigf, double phipf, double (&acc)[8])
+{
+    const int slot[8] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP};
+    const double tgf = fv[3];
+    const double amug = P.as * sqrt(tgf) / (1.0 + P.ts / tgf);
+    const double akg = amug * P.cpg / P.prg;
+    const double amup = 0.001;
+    const double akp = amup * P.cpp / P.prp;
+    double dx[NM], dy[NM], dz[NM];
+#pragma unroll
+    for (int k = 0; k < NM; k++) {
+        dx[k] = m.xc[mm[k]] - xf;
+        dy[k] = m.yc[mm[k]] - yf;
+        dz[k] = m.zc[mm[k]] - zf;
+    }
+    double a11 = dx[0] * dx[0], a22 = dy[0] * dy[0], a33 = dz[0] * dz[0];
+    double a12 = dx[0] * dy[0], a13 = dx[0] * dz[0], a23 = dy[0] * dz[0];
+#pragma unroll
+    for (int k = 1; k < NM; k++) {
+        a11 = a11 + dx[k] * dx[k];
+        a22 = a22 + dy[k] * dy[k];
+        a33 = a33 + dz[k] * dz[k];
+        a12 = a12 + dx[k] * dy[k];
+        a13 = a13 + dx[k] * dz[k];
+        a23 = a23 + dy[k] * dz[k];
+    }
+    const double dnr = det3(a11, a12, a13, a12, a22, a23, a13, a23, a33);
+#pragma unroll
+    for (int ph = 0; ph < 2; ph++) {
+        const int o = 4 * ph;
+        double gr[4][3];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double dv = QV(slot[o + v], mm[0]) - fv[o + v];
+            double b1 = dv * dx[0], b2 = dv * dy[0], b3 = dv * dz[0];
+#pragma unroll
+            for (int k = 1; k < NM; k++) {
+                dv = QV(slot[o + v], mm[k]) - fv[o + v];
+                b1 = b1 + dv * dx[k];
+                b2 = b2 + dv * dy[k];
+                b3 = b3 + dv * dz[k];
+            }
+            gr[v][0] = det3(b1, b2, b3, a12, a22, a23, a13, a23, a33) / dnr;
+            gr[v][1] = det3(a11, a12, a13, b1, b2, b3, a13, a23, a33) / dnr;
+            gr[v][2] = det3(a11, a12, a13, a12, a22, a23, b1, b2, b3) / dnr;
+        }
+        // Newtonian stress and Fourier heat flux (:761-804)
+        const double mu = ph ? amup : amug, ak = ph ? akp : akg;
+        const double uf = fv[o], vf = fv[o + 1], wf = fv[o + 2];
+        const double dux = gr[0][0], duy = gr[0][1], duz = gr[0][2];
+        const double dvx = gr[1][0], dvy = gr[1][1], dvz = gr[1][2];
+        const double dwx = gr[2][0], dwy = gr[2][1], dwz = gr[2][2];
+        const double dtx = gr[3][0], dty = gr[3][1], dtz = gr[3][2];
+        const double delv = dux + dvy + dwz;
+        const double txx = mu * (2.0 * dux - P.c23 * delv);
+        const double tyy = mu * (2.0 * dvy - P.c23 * delv);
+        const double tzz = mu * (2.0 * dwz - P.c23 * delv);
+        const double txy = mu * (dvx + duy);
+        const double txz = mu * (dwx + duz);
+        const double tyz = mu * (dvz + dwy);
+        const double qxx = uf * txx + vf * txy + wf * txz + ak * dtx;
+        const double qyy = uf * txy + vf * tyy + wf * tyz + ak * dty;
+        const double qzz = uf * txz + vf * tyz + wf * tzz + ak * dtz;
+        const double phf = ph ? phipf : phigf;
+        acc[o + 0] = phf * (txx * scx + txy * scy + txz * scz);
+        acc[o + 1] = phf * (txy * scx + tyy * scy + tyz * scz);
+        acc[o + 2] = phf * (txz * scx + tyz * scy + tzz * scz);
+        acc[o + 3] = phf * (qxx * scx + qyy * scy + qzz * scz);
+    }
+}
+
 // One cell per quad, one face per lane; the quad sums the four face
 // contributions in face order, vis = (((0+V1)+V2)+V3)+V4, as the reference's
 // accumulation into zeroed arrays (sr_viscousgp.F90:41-48, :797-804).
@@ -524,76 +604,15 @@ __device__ __forceinline__ void viscous_face(const Params &P, const State &s, co
         for (int v = 0; v < 8; v++) fv[v] = (QV(slot[v], i) + QV(slot[v], neigh)) * 0.5;
         const double phigf = (QV(Q_PHIG, i) + QV(Q_PHIG, neigh)) * 0.5;
         const double phipf = (QV(Q_PHIP, i) + QV(Q_PHIP, neigh)) * 0.5;
-        const double tgf = fv[3];
-        const double amug = P.as * sqrt(tgf) / (1.0 + P.ts / tgf);
-        const double akg = amug * P.cpg / P.prg;
-        const double amup = 0.001;
-        const double akp = amup * P.cpp / P.prp;
         int mm[8];
-        int nm;
         for (int k = 0; k < 4; k++) mm[k] = m.nc[(size_t)k * ne + i];
-        if (neigh >= ne) {  // boundary face: the cell's 4 neighbours and the cell
+        if (neigh >= ne) {
             mm[4] = i;
-            nm = 5;
-        } else {            // interior face: plus the neighbour's 4 neighbours
+            mm[5] = mm[6] = mm[7] = i;
+            viscous_stencil<5>(P, q, nt, m, mm, xf, yf, zf, scx, scy, scz, fv, phigf, phipf, acc);
+        } else {
             for (int k = 0; k < 4; k++) mm[4 + k] = m.nc[(size_t)k * ne + neigh];
-            nm = 8;
-        }
-        double dx[8], dy[8], dz[8];
-        for (int k = 0; k < nm; k++) {
-            dx[k] = m.xc[mm[k]] - xf;
-            dy[k] = m.yc[mm[k]] - yf;
-            dz[k] = m.zc[mm[k]] - zf;
-        }
-        double a11 = dx[0] * dx[0], a22 = dy[0] * dy[0], a33 = dz[0] * dz[0];
-        double a12 = dx[0] * dy[0], a13 = dx[0] * dz[0], a23 = dy[0] * dz[0];
-        for (int k = 1; k < nm; k++) {
-            a11 = a11 + dx[k] * dx[k];
-            a22 = a22 + dy[k] * dy[k];
-            a33 = a33 + dz[k] * dz[k];
-            a12 = a12 + dx[k] * dy[k];
-            a13 = a13 + dx[k] * dz[k];
-            a23 = a23 + dy[k] * dz[k];
-        }
-        const double dnr = det3(a11, a12, a13, a12, a22, a23, a13, a23, a33);
-        for (int ph = 0; ph < 2; ph++) {
-            const int o = 4 * ph;
-            double gr[4][3];
-            for (int v = 0; v < 4; v++) {
-                double dv = QV(slot[o + v], mm[0]) - fv[o + v];
-                double b1 = dv * dx[0], b2 = dv * dy[0], b3 = dv * dz[0];
-                for (int k = 1; k < nm; k++) {
-                    dv = QV(slot[o + v], mm[k]) - fv[o + v];
-                    b1 = b1 + dv * dx[k];
-                    b2 = b2 + dv * dy[k];
-                    b3 = b3 + dv * dz[k];
-                }
-                gr[v][0] = det3(b1, b2, b3, a12, a22, a23, a13, a23, a33) / dnr;
-                gr[v][1] = det3(a11, a12, a13, b1, b2, b3, a13, a23, a33) / dnr;
-                gr[v][2] = det3(a11, a12, a13, a12, a22, a23, b1, b2, b3) / dnr;
-            }
-            // Newtonian stress and Fourier heat flux (:761-804)
-            const double mu = ph ? amup : amug, ak = ph ? akp : akg;
-            const double uf = fv[o], vf = fv[o + 1], wf = fv[o + 2];
-            const double dux = gr[0][0], duy = gr[0][1], duz = gr[0][2];
-            const double dvx = gr[1][0], dvy = gr[1][1], dvz = gr[1][2];
-            const double dwx = gr[2][0], dwy = gr[2][1], dwz = gr[2][2];
-            const double dtx = gr[3][0], dty = gr[3][1], dtz = gr[3][2];
-            const double delv = dux + dvy + dwz;
-            const double txx = mu * (2.0 * dux - P.c23 * delv);
-            const double tyy = mu * (2.0 * dvy - P.c23 * delv);
-            const double tzz = mu * (2.0 * dwz - P.c23 * delv);
-            const double txy = mu * (dvx + duy);
-            const double txz = mu * (dwx + duz);
-            const double tyz = mu * (dvz + dwy);
-            const double qxx = uf * txx + vf * txy + wf * txz + ak * dtx;
-            const double qyy = uf * txy + vf * tyy + wf * tyz + ak * dty;
-            const double qzz = uf * txz + vf * tyz + wf * tzz + ak * dtz;
-            const double phf = ph ? phipf : phigf;
-            acc[o + 0] = phf * (txx * scx + txy * scy + txz * scz);
-            acc[o + 1] = phf * (txy * scx + tyy * scy + tyz * scz);
-            acc[o + 2] = phf * (txz * scx + tyz * scy + tzz * scz);
-            acc[o + 3] = phf * (qxx * scx + qyy * scy + qzz * scz);
+            viscous_stencil<8>(P, q, nt, m, mm, xf, yf, zf, scx, scy, scz, fv, phigf, phipf, acc);
         }
     }
     for (int v = 0; v < 8; v++) {
@@ -1637,11 +1656,79 @@ __device__ __forceinline__ void flux_face_f32(const Params &P, const State &s, c
     source_and_update_f32(P, s, m, i, f, qbase, neigh, rhs, stage);
 }
 
+// viscous_stencil in FP32: differences of FP64 fields are formed in FP64
+// and rounded once; sums, determinants and stresses run in FP32.
+template <int NM>
+__device__ __forceinline__ void viscous_stencil_f32(const Params &P, const double *q, size_t nt, const Mesh &m,
+                                                    const int (&mm)[8], double xf, double yf, double zf,
+                                                    float scx, float scy, float scz, const double (&fv)[8],
+                                                    float phigf, float phipf, float (&acc)[8])
+{
+#pragma clang fp contract(fast)
+    const int slot[8] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP};
+    const float tgf = (float)fv[3];
+    const float amug = (float)P.as * fsqr(tgf) * frcp(1.0f + (float)P.ts * frcp(tgf));
+    const float akg = amug * (float)(P.cpg / P.prg);
+    const float amup = 0.001f;
+    const float akp = amup * (float)(P.cpp / P.prp);
+    const float c23 = (float)P.c23;
+    float dx[NM], dy[NM], dz[NM];
+    float a11 = 0.0f, a22 = 0.0f, a33 = 0.0f, a12 = 0.0f, a13 = 0.0f, a23 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NM; k++) {
+        dx[k] = (float)(m.xc[mm[k]] - xf);
+        dy[k] = (float)(m.yc[mm[k]] - yf);
+        dz[k] = (float)(m.zc[mm[k]] - zf);
+        a11 += dx[k] * dx[k];
+        a22 += dy[k] * dy[k];
+        a33 += dz[k] * dz[k];
+        a12 += dx[k] * dy[k];
+        a13 += dx[k] * dz[k];
+        a23 += dy[k] * dz[k];
+    }
+    const float idn = frcp(det3f(a11, a12, a13, a12, a22, a23, a13, a23, a33));
+#pragma unroll
+    for (int ph = 0; ph < 2; ph++) {
+        const int o = 4 * ph;
+        float gr[4][3];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            float b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NM; k++) {
+                const float dv = (float)(QV(slot[o + v], mm[k]) - fv[o + v]);
+                b1 += dv * dx[k];
+                b2 += dv * dy[k];
+                b3 += dv * dz[k];
+            }
+            gr[v][0] = det3f(b1, b2, b3, a12, a22, a23, a13, a23, a33) * idn;
+            gr[v][1] = det3f(a11, a12, a13, b1, b2, b3, a13, a23, a33) * idn;
+            gr[v][2] = det3f(a11, a12, a13, a12, a22, a23, b1, b2, b3) * idn;
+        }
+        const float mu = ph ? amup : amug, ak = ph ? akp : akg;
+        const float uf = (float)fv[o], vf = (float)fv[o + 1], wf = (float)fv[o + 2];
+        const float delv = gr[0][0] + gr[1][1] + gr[2][2];
+        const float txx = mu * (2.0f * gr[0][0] - c23 * delv);
+        const float tyy = mu * (2.0f * gr[1][1] - c23 * delv);
+        const float tzz = mu * (2.0f * gr[2][2] - c23 * delv);
+        const float txy = mu * (gr[1][0] + gr[0][1]);
+        const float txz = mu * (gr[2][0] + gr[0][2]);
+        const float tyz = mu * (gr[1][2] + gr[2][1]);
+        const float qxx = uf * txx + vf * txy + wf * txz + ak * gr[3][0];
+        const float qyy = uf * txy + vf * tyy + wf * tyz + ak * gr[3][1];
+        const float qzz = uf * txz + vf * tyz + wf * tzz + ak * gr[3][2];
+        const float phf = ph ? phipf : phigf;
+        acc[o + 0] = phf * (txx * scx + txy * scy + txz * scz);
+        acc[o + 1] = phf * (txy * scx + tyy * scy + tyz * scz);
+        acc[o + 2] = phf * (txz * scx + tyz * scy + tzz * scz);
+        acc[o + 3] = phf * (qxx * scx + qyy * scy + qzz * scz);
+    }
+}
+
 // viscous face flux (sr_viscousgp.F90:450-1139) in FP32, one face per lane
 __device__ __forceinline__ void viscous_face_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
                                                  int qbase)
 {
-#pragma clang fp contract(fast)
     const int ne = m.ne;
     if (m.iblank[i] != 1) return;
     const double *q = s.q;
@@ -1658,68 +1745,15 @@ __device__ __forceinline__ void viscous_face_f32(const Params &P, const State &s
         for (int v = 0; v < 8; v++) fv[v] = (QV(slot[v], i) + QV(slot[v], neigh)) * 0.5;
         const float phigf = (float)((QV(Q_PHIG, i) + QV(Q_PHIG, neigh)) * 0.5);
         const float phipf = (float)((QV(Q_PHIP, i) + QV(Q_PHIP, neigh)) * 0.5);
-        const float tgf = (float)fv[3];
-        const float amug = (float)P.as * fsqr(tgf) * frcp(1.0f + (float)P.ts * frcp(tgf));
-        const float akg = amug * (float)(P.cpg / P.prg);
-        const float amup = 0.001f;
-        const float akp = amup * (float)(P.cpp / P.prp);
-        const float c23 = (float)P.c23;
         int mm[8];
-        int nm;
         for (int k = 0; k < 4; k++) mm[k] = m.nc[(size_t)k * ne + i];
         if (neigh >= ne) {
             mm[4] = i;
-            nm = 5;
+            mm[5] = mm[6] = mm[7] = i;
+            viscous_stencil_f32<5>(P, q, nt, m, mm, xf, yf, zf, scx, scy, scz, fv, phigf, phipf, acc);
         } else {
             for (int k = 0; k < 4; k++) mm[4 + k] = m.nc[(size_t)k * ne + neigh];
-            nm = 8;
-        }
-        float dx[8], dy[8], dz[8];
-        float a11 = 0.0f, a22 = 0.0f, a33 = 0.0f, a12 = 0.0f, a13 = 0.0f, a23 = 0.0f;
-        for (int k = 0; k < nm; k++) {
-            dx[k] = (float)(m.xc[mm[k]] - xf);
-            dy[k] = (float)(m.yc[mm[k]] - yf);
-            dz[k] = (float)(m.zc[mm[k]] - zf);
-            a11 += dx[k] * dx[k];
-            a22 += dy[k] * dy[k];
-            a33 += dz[k] * dz[k];
-            a12 += dx[k] * dy[k];
-            a13 += dx[k] * dz[k];
-            a23 += dy[k] * dz[k];
-        }
-        const float idn = frcp(det3f(a11, a12, a13, a12, a22, a23, a13, a23, a33));
-        for (int ph = 0; ph < 2; ph++) {
-            const int o = 4 * ph;
-            float gr[4][3];
-            for (int v = 0; v < 4; v++) {
-                float b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
-                for (int k = 0; k < nm; k++) {
-                    const float dv = (float)(QV(slot[o + v], mm[k]) - fv[o + v]);
-                    b1 += dv * dx[k];
-                    b2 += dv * dy[k];
-                    b3 += dv * dz[k];
-                }
-                gr[v][0] = det3f(b1, b2, b3, a12, a22, a23, a13, a23, a33) * idn;
-                gr[v][1] = det3f(a11, a12, a13, b1, b2, b3, a13, a23, a33) * idn;
-                gr[v][2] = det3f(a11, a12, a13, a12, a22, a23, b1, b2, b3) * idn;
-            }
-            const float mu = ph ? amup : amug, ak = ph ? akp : akg;
-            const float uf = (float)fv[o], vf = (float)fv[o + 1], wf = (float)fv[o + 2];
-            const float delv = gr[0][0] + gr[1][1] + gr[2][2];
-            const float txx = mu * (2.0f * gr[0][0] - c23 * delv);
-            const float tyy = mu * (2.0f * gr[1][1] - c23 * delv);
-            const float tzz = mu * (2.0f * gr[2][2] - c23 * delv);
-            const float txy = mu * (gr[1][0] + gr[0][1]);
-            const float txz = mu * (gr[2][0] + gr[0][2]);
-            const float tyz = mu * (gr[1][2] + gr[2][1]);
-            const float qxx = uf * txx + vf * txy + wf * txz + ak * gr[3][0];
-            const float qyy = uf * txy + vf * tyy + wf * tyz + ak * gr[3][1];
-            const float qzz = uf * txz + vf * tyz + wf * tzz + ak * gr[3][2];
-            const float phf = ph ? phipf : phigf;
-            acc[o + 0] = phf * (txx * scx + txy * scy + txz * scz);
-            acc[o + 1] = phf * (txy * scx + tyy * scy + tyz * scz);
-            acc[o + 2] = phf * (txz * scx + tyz * scy + tzz * scz);
-            acc[o + 3] = phf * (qxx * scx + qyy * scy + qzz * scz);
+            viscous_stencil_f32<8>(P, q, nt, m, mm, xf, yf, zf, scx, scy, scz, fv, phigf, phipf, acc);
         }
     }
     for (int v = 0; v < 8; v++) {
