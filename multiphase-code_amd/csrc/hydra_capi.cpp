@@ -322,6 +322,8 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     ALLOC(s.anu, 2 * (size_t)ne) ALLOC(s.rhs, 10 * (size_t)ne) ALLOC(s.scal, SC_N) ALLOC(s.itot, 1)
     if (params->precision == 1) {
         ALLOC(s.g32, (size_t)3 * G_N * ne) ALLOC(s.phi32, ne)
+    } else {
+        ALLOC(s.src, (size_t)SRC_N * ne)
     }
 #undef ALLOC
     m.x = dx; m.y = dy; m.z = dz; m.nod = dnod; m.nc = dnc; m.ncp = dncp; m.iblank = dib;
@@ -965,7 +967,8 @@ size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision)
     b += (13 * ne + 4 * ng) * I;                             // nod, nc, ncp, iblank, ghost lists
     b += (12 + 3 + 12 + 12) * ne * D + 4 * nt * D;         // sc, dl/epssq/dd, fc0, fc3, xc..vol
     b += (30 + 3 * G_N + 1 + 8 + 10 + 10 + 2 + 10) * ne * D + (size_t)Q_N * nt * D + SC_N * D + I;
-    if (precision == 1) b += (3 * G_N + 1) * ne * sizeof(float);
+    if (precision == 1) b += (3 * G_N + 1) * ne * sizeof(float);  // FP32 gradients, limiter
+    else b += SRC_N * ne * D;                                     // interphase source terms
     const size_t tc = (ne + TILE - 1) / TILE, tg = (ng + TILE - 1) / TILE + 1;
     b += sizeof(Params) + sizeof(State) + sizeof(Mesh) + (tc + tg) * sizeof(Tile) + 4 * I + 4 * tc * D;
     return b + 64 * 4096;  // allocation granularity of the ~60 buffers

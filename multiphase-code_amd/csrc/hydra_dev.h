@@ -13,6 +13,10 @@ typedef hp_params Params;
 
 // primitive slots of q (each ntot long: interior cells, then ghosts)
 enum { Q_UG, Q_VG, Q_WG, Q_PG, Q_PP, Q_TG, Q_PHIG, Q_UP, Q_VP, Q_WP, Q_PHIP, Q_TP, Q_ROP, Q_ROG, Q_N };
+// interphase source terms x cell volume, per stage (sr_source.F90:46-155):
+// gas x/y/z momentum and energy (the liquid takes y/z momentum and energy
+// with the opposite sign) and liquid x momentum, whose gravity term differs
+enum { SRC_MX, SRC_MY, SRC_MZ, SRC_E, SRC_MXP, SRC_N };
 // stored gradient slots (x, y, z each)
 enum { G_UG, G_VG, G_WG, G_TG, G_UP, G_VP, G_WP, G_TP, G_PG, G_PP, G_PHIG, G_N };
 // device scalars
@@ -41,6 +45,7 @@ struct State {
     double *q;              // Q_N x nt
     double *grad;           // 3*G_N x ne
     double *phi;            // ne
+    double *src;            // SRC_N x ne interphase source terms x volume (FP64 form; else null)
     float *g32, *phi32;     // FP32-bulk variant: gradients and limiter (else null)
     double *vis;            // 8 x ne
     double *d24, *d2q;      // 10 x ne

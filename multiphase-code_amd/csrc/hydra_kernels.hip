@@ -627,6 +627,88 @@ __device__ __forceinline__ void viscous_face(const Params &P, const State &s, co
 // the 4 face neighbours, Cramer's rule) fused with the Venkatakrishnan
 // limiter (sr_VK.F90:32-274), which only needs this cell's gradients.
 // ---------------------------------------------------------------------------
+// Interphase source of cell i (sr_source.F90:46-155) from the stage's
+// primitives: drag, interface pressure times the gradient of phig (an LSQ fit
+// over the four face neighbours with 1/distance weights) and heat exchange.
+// Stored as the five terms times the cell volume that the stage update
+// subtracts (gas) or adds (liquid), each formed as the reference forms it.
+__device__ __forceinline__ void source_cell(const Params &P, const State &s, const Mesh &m, int i)
+{
+    const int ne = m.ne;
+    if (m.iblank[i] != 1) return;
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    int nb[4];
+    for (int k = 0; k < 4; k++) nb[k] = m.nc[(size_t)k * ne + i];
+    const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i];
+    double dx[4], dy[4], dz[4], wt[4];
+    for (int k = 0; k < 4; k++) {
+        dx[k] = m.xc[nb[k]] - xi;
+        dy[k] = m.yc[nb[k]] - yi;
+        dz[k] = m.zc[nb[k]] - zi;
+        wt[k] = dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k];
+    }
+    const double phig = QV(Q_PHIG, i), phip = QV(Q_PHIP, i), rog = QV(Q_ROG, i), rop = QV(Q_ROP, i);
+    const double ug = QV(Q_UG, i), vg = QV(Q_VG, i), wg = QV(Q_WG, i), up = QV(Q_UP, i), vp = QV(Q_VP, i),
+                 wp = QV(Q_WP, i), tg = QV(Q_TG, i), tp = QV(Q_TP, i), pg = QV(Q_PG, i);
+    const double voli = m.vol[i], dl = m.dl[i];
+    const double amug = P.as * sqrt(tg) / (1.0 + P.ts / tg);
+    const double du = ug - up, dv = vg - vp, dw = wg - wp;
+    const double drag = 0.44 * phig * (phig - 1.0) * rog * sqrt(du * du + dv * dv + dw * dw) * dl;
+    const double fdgx = drag * du, fdgy = drag * dv, fdgz = drag * dw;
+    const double qsource = 6.0 * phip * amug * P.cpg * 2.0 * (tp - tg) / (P.prg * (dl * dl));
+    const double delpstar = 2.0 * (phig * rog * phip * rop) / (phig * rop + phip * rog) * (du * du + dv * dv + dw * dw);
+    const double pint = pg - dmin(delpstar, 0.01 * pg);
+    const double den = phig * rog + (1.0 - phig) * rop;
+    const Divisor DN = divisor(den);
+    const double uint = divide(phig * rog * ug + (1.0 - phig) * rop * up, DN);
+    const double vint = divide(phig * rog * vg + (1.0 - phig) * rop * vp, DN);
+    const double wint = divide(phig * rog * wg + (1.0 - phig) * rop * wp, DN);
+    // face order k = 0..3, summed left to right
+    Divisor W[4];
+    double dph[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        W[k] = divisor(sqrt(wt[k]));
+        dph[k] = QV(Q_PHIG, nb[k]) - phig;
+    }
+#define SUM4(ex) (divide(ex(0), W[0]) + divide(ex(1), W[1]) + divide(ex(2), W[2]) + divide(ex(3), W[3]))
+#define XX(k) (dx[k] * dx[k])
+#define YY(k) (dy[k] * dy[k])
+#define ZZ(k) (dz[k] * dz[k])
+#define XY(k) (dx[k] * dy[k])
+#define XZ(k) (dx[k] * dz[k])
+#define YZ(k) (dy[k] * dz[k])
+#define PX(k) (dph[k] * dx[k])
+#define PY(k) (dph[k] * dy[k])
+#define PZ(k) (dph[k] * dz[k])
+    const double a1 = SUM4(XX), a2 = SUM4(YY), a3 = SUM4(ZZ), b1 = SUM4(XY), b2 = SUM4(XZ), b3 = SUM4(YZ);
+    const double c1 = SUM4(PX), c2 = SUM4(PY), c3 = SUM4(PZ);
+#undef SUM4
+#undef XX
+#undef YY
+#undef ZZ
+#undef XY
+#undef XZ
+#undef YZ
+#undef PX
+#undef PY
+#undef PZ
+    const double dnr = det3(a1, b1, b2, b1, a2, b3, b2, b3, a3);
+    const Divisor D = divisor(dnr);
+    const double dphx = divide(det3(c1, c2, c3, b1, a2, b3, b2, b3, a3), D);
+    const double dphy = divide(det3(a1, b1, b2, c1, c2, c3, b2, b3, a3), D);
+    const double dphz = divide(det3(a1, b1, b2, b1, a2, b3, c1, c2, c3), D);
+    const double work = pint * (uint * dphx + vint * dphy + wint * dphz) + uint * fdgx + vint * fdgy +
+                        wint * fdgz + qsource;
+    double *o = s.src;
+    o[(size_t)SRC_MX * ne + i] = (fdgx + pint * dphx + phig * rog * P.gravity) * voli;
+    o[(size_t)SRC_MY * ne + i] = (fdgy + pint * dphy) * voli;
+    o[(size_t)SRC_MZ * ne + i] = (fdgz + pint * dphz) * voli;
+    o[(size_t)SRC_E * ne + i] = work * voli;
+    o[(size_t)SRC_MXP * ne + i] = (fdgx + pint * dphx - phip * rop * P.gravity) * voli;
+}
+
 __device__ __forceinline__ void grad_cell(const State &s, const Mesh &m, int i)
 {
     const int ne = m.ne;
@@ -978,68 +1060,28 @@ __device__ __forceinline__ void rk_finish(const State &s, size_t o, double r, in
     s.cn[o] = cn;
 }
 
-// Interphase source (sr_source.F90:46-155) and the RK stage update
-// (sr_RK.F90:29-161) of the quad's cell i from its face-summed residual
-// (FP64 in every precision).  Lane f of the quad finishes the conserved
-// variables k = f, f+4, f+8, whose face sums are rhs[0], rhs[1], rhs[2].  stage >= 1: update cn (and cn1); stage == 0:
-// store the residual in rhs (per-routine checks).
-__device__ __forceinline__ void source_and_update(const Params &P, const State &s, const Mesh &m, int i, int f,
-                                                  int qbase, int neigh, const double *rhs, int stage)
+// RK stage update (sr_RK.F90:29-161) of the quad's cell i from its
+// face-summed residual and the stored interphase source terms (source_cell;
+// FP64 form).  Lane f of the quad finishes the conserved variables k = f,
+// f+4, f+8, whose face sums are rhs[0], rhs[1], rhs[2].  stage >= 1: update
+// cn (and cn1); stage == 0: store the residual in rhs (per-routine checks).
+__device__ __forceinline__ void source_and_update(const State &s, const Mesh &m, int i, int f,
+                                                  const double *rhs, int stage)
 {
     const int ne = m.ne;
-    const double *q = s.q;
-    const size_t nt = s.nt;
-    const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i];
-    // interphase source (sr_source.F90:46-155)
-    const double phig = QV(Q_PHIG, i), phip = QV(Q_PHIP, i), rog = QV(Q_ROG, i), rop = QV(Q_ROP, i);
-    const double ug = QV(Q_UG, i), vg = QV(Q_VG, i), wg = QV(Q_WG, i), up = QV(Q_UP, i), vp = QV(Q_VP, i),
-                 wp = QV(Q_WP, i), tg = QV(Q_TG, i), tp = QV(Q_TP, i), pg = QV(Q_PG, i);
-    const double voli = m.vol[i], dl = m.dl[i];
-    const double amug = P.as * sqrt(tg) / (1.0 + P.ts / tg);
-    const double du = ug - up, dv = vg - vp, dw = wg - wp;
-    const double drag = 0.44 * phig * (phig - 1.0) * rog * sqrt(du * du + dv * dv + dw * dw) * dl;
-    const double fdgx = drag * du, fdgy = drag * dv, fdgz = drag * dw;
-    const double qsource = 6.0 * phip * amug * P.cpg * 2.0 * (tp - tg) / (P.prg * (dl * dl));
-    const double delpstar = 2.0 * (phig * rog * phip * rop) / (phig * rop + phip * rog) * (du * du + dv * dv + dw * dw);
-    const double pint = pg - dmin(delpstar, 0.01 * pg);
-    const double den = phig * rog + (1.0 - phig) * rop;
-    const double uint = (phig * rog * ug + (1.0 - phig) * rop * up) / den;
-    const double vint = (phig * rog * vg + (1.0 - phig) * rop * vp) / den;
-    const double wint = (phig * rog * wg + (1.0 - phig) * rop * wp) / den;
-    double a1, a2, a3, b1, b2, b3, c1, c2, c3;
-    {
-        const double dx = m.xc[neigh] - xi, dy = m.yc[neigh] - yi, dz = m.zc[neigh] - zi;
-        const double dph = QV(Q_PHIG, neigh) - phig;
-        const double wt = sqrt(dx * dx + dy * dy + dz * dz);
-        a1 = quad_sum4(dx * dx / wt, qbase);
-        a2 = quad_sum4(dy * dy / wt, qbase);
-        a3 = quad_sum4(dz * dz / wt, qbase);
-        b1 = quad_sum4(dx * dy / wt, qbase);
-        b2 = quad_sum4(dx * dz / wt, qbase);
-        b3 = quad_sum4(dy * dz / wt, qbase);
-        c1 = quad_sum4(dph * dx / wt, qbase);
-        c2 = quad_sum4(dph * dy / wt, qbase);
-        c3 = quad_sum4(dph * dz / wt, qbase);
-    }
-    const double dnr = det3(a1, b1, b2, b1, a2, b3, b2, b3, a3);
-    const double dphx = det3(c1, c2, c3, b1, a2, b3, b2, b3, a3) / dnr;
-    const double dphy = det3(a1, b1, b2, c1, c2, c3, b2, b3, a3) / dnr;
-    const double dphz = det3(a1, b1, b2, b1, a2, b3, c1, c2, c3) / dnr;
-    const double work = pint * (uint * dphx + vint * dphy + wint * dphz) + uint * fdgx + vint * fdgy +
-                        wint * fdgz + qsource;
-    const double dtbyvol = (stage > 0) ? s.scal[SC_DT] / voli : 0.0;
+    const double *vis = s.vis, *src = s.src;
+    const double dtbyvol = (stage > 0) ? s.scal[SC_DT] / m.vol[i] : 0.0;
     for (int k = f; k < 10; k += 4) {
         double r = rhs[k >> 2];
-        const double *vis = s.vis;
         switch (k) {
-        case 1: r = r - (fdgx + pint * dphx + phig * rog * P.gravity) * voli - vis[(size_t)0 * ne + i]; break;
-        case 2: r = r - (fdgy + pint * dphy) * voli - vis[(size_t)1 * ne + i]; break;
-        case 3: r = r - (fdgz + pint * dphz) * voli - vis[(size_t)2 * ne + i]; break;
-        case 4: r = r - work * voli - vis[(size_t)3 * ne + i]; break;
-        case 6: r = r + (fdgx + pint * dphx - phip * rop * P.gravity) * voli - vis[(size_t)4 * ne + i]; break;
-        case 7: r = r + (fdgy + pint * dphy) * voli - vis[(size_t)5 * ne + i]; break;
-        case 8: r = r + (fdgz + pint * dphz) * voli - vis[(size_t)6 * ne + i]; break;
-        case 9: r = r + work * voli - vis[(size_t)7 * ne + i]; break;
+        case 1: r = r - src[(size_t)SRC_MX * ne + i] - vis[(size_t)0 * ne + i]; break;
+        case 2: r = r - src[(size_t)SRC_MY * ne + i] - vis[(size_t)1 * ne + i]; break;
+        case 3: r = r - src[(size_t)SRC_MZ * ne + i] - vis[(size_t)2 * ne + i]; break;
+        case 4: r = r - src[(size_t)SRC_E * ne + i] - vis[(size_t)3 * ne + i]; break;
+        case 6: r = r + src[(size_t)SRC_MXP * ne + i] - vis[(size_t)4 * ne + i]; break;
+        case 7: r = r + src[(size_t)SRC_MY * ne + i] - vis[(size_t)5 * ne + i]; break;
+        case 8: r = r + src[(size_t)SRC_MZ * ne + i] - vis[(size_t)6 * ne + i]; break;
+        case 9: r = r + src[(size_t)SRC_E * ne + i] - vis[(size_t)7 * ne + i]; break;
         default: break;
         }
         rk_finish(s, (size_t)k * ne + i, r, stage, dtbyvol);
@@ -1115,7 +1157,7 @@ __device__ __forceinline__ void flux_face(const Params &P, const State &s, const
         const double sum = 0.0 + t0 + t1 + t2 + t3;
         if ((k & 3) == f) rhs[k >> 2] = sum;
     }
-    source_and_update(P, s, m, i, f, qbase, neigh, rhs, stage);
+    source_and_update(s, m, i, f, rhs, stage);
 }
 
 // stage update from a stored residual (sr_RK.F90), for the per-routine path
@@ -1869,17 +1911,25 @@ __global__ void __launch_bounds__(TILE) k_smooth2(Batch B)
     }
 }
 
-template <int MODE, bool F32>
-__global__ void __launch_bounds__(TILE) k_grad_limiter(Batch B)
+// F32: FP32-bulk gradients and limiter (one wave per tile).  FP64: GRAD
+// runs two waves per tile, the first computing gradients and limiter, the
+// second the tile's interphase source terms, so that neither path's
+// registers add to the other's; without GRAD (first order) one wave computes
+// the source.
+template <int MODE, bool F32, bool GRAD>
+__global__ void __launch_bounds__((!F32 && GRAD) ? 2 * TILE : TILE) k_grad_limiter(Batch B)
 {
     TILE_LOOP(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
-        if ((int)threadIdx.x < C.count) {
+        const int lane = (int)(threadIdx.x & 63u);
+        if (lane < C.count) {
             if (F32)
-                grad_cell_f32(C.S, C.M, C.first + (int)threadIdx.x);
+                grad_cell_f32(C.S, C.M, C.first + lane);
+            else if (GRAD && threadIdx.x < TILE)
+                grad_cell(C.S, C.M, C.first + lane);
             else
-                grad_cell(C.S, C.M, C.first + (int)threadIdx.x);
+                source_cell(C.P, C.S, C.M, C.first + lane);
         }
     }
 }
@@ -2002,11 +2052,14 @@ void launch_viscous(const Batch &B, const Grid &G, hipStream_t st)
 
 void launch_grad(const Batch &B, const Grid &G, hipStream_t st)
 {
-    if (G.myorder != 2) return;
-    if (G.precision == 1)
-        hipLaunchKernelGGL(PICK(1, k_grad_limiter, true), dim3(G.cell), dim3(TILE), 0, st, B);
-    else
-        hipLaunchKernelGGL(PICK(1, k_grad_limiter, false), dim3(G.cell), dim3(TILE), 0, st, B);
+    if (G.precision == 1) {  // the FP32-bulk flux forms its source itself
+        if (G.myorder == 2)
+            hipLaunchKernelGGL(PICK(1, k_grad_limiter, true, true), dim3(G.cell), dim3(TILE), 0, st, B);
+    } else if (G.myorder == 2) {
+        hipLaunchKernelGGL(PICK(1, k_grad_limiter, false, true), dim3(G.cell), dim3(2 * TILE), 0, st, B);
+    } else {
+        hipLaunchKernelGGL(PICK(1, k_grad_limiter, false, false), dim3(G.cell), dim3(TILE), 0, st, B);
+    }
 }
 
 void launch_flux(const Batch &B, const Grid &G, int stage, hipStream_t st)

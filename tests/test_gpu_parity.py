@@ -195,6 +195,25 @@ def test_steps_bitwise(flux, fused):
     assert_same("ghost prims", gp, gpr)
 
 
+@pytest.mark.parametrize("flux", [2, 1])
+def test_first_order_steps_bitwise(flux):
+    """myorder = 1: no reconstruction; the interphase source still comes
+    from its own kernel (no gradients or limiter are computed)."""
+    m = M.box_mesh(6, 5, 4, jitter=0.1, seed=5, hole=(0.05, 0.0, 0.16))
+    p = M.water_air_params(cfl=0.5, viscous_gas=True, myflux=flux)
+    p["myorder"] = 1
+    cu0, gp0 = M.sphere_state(m, p, [(0.0, 0.1, 0.0, 0.22)])
+    o = O.Oracle(m, p)
+    o.set_state(cu0, gp0, 0.0, 0)
+    s = solver_cls()(m, p)
+    s.upload(cu0, gp0, 0.0, 0)
+    for fused in (True, False):
+        (cu, gp, t, tot), (cur, gpr, tr, totr) = run_pair(m, p, o, s, 3, fused=fused)
+        assert tot == totr and t == tr
+        assert_same(f"cu fused={fused}", cu, cur)
+        assert_same(f"ghost prims fused={fused}", gp, gpr)
+
+
 def test_config1_100_steps():
     """Configuration 1 workload: 24^3 hexes x 6 = 82,944 tets, jittered,
     water sphere R=0.2 in an air stream at Mach 0.3, CFL 0.5, 100 steps.
