@@ -21,6 +21,7 @@ stages) per second with the state resident in HBM.
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
        python bench.py summarize-counters FETCH.csv WRITE.csv CALIB_MB
        python bench.py summarize-trace results.db
+       python bench.py summarize-instructions counter_collection.csv
 
 roofline.traffic is the HBM bytes of one k_flux launch measured with
 rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (each run with
@@ -157,6 +158,29 @@ def summarize_counters(fetch_csv, write_csv, calib_mb):
         print(f"\ntraffic_bytes_per_launch k_flux: {flux}")
 
 
+def summarize_instructions(sq_csv):
+    """Per-kernel means per dispatch of one rocprofv3 --pmc run of SQ_WAVES,
+    SQ_INSTS_VALU, SQ_INSTS_VMEM_RD, SQ_INSTS_LDS, SQ_ACTIVE_INST_VALU and
+    SQ_WAVE_CYCLES.  Prints markdown."""
+    import csv
+    import collections
+    tot = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(set)
+    with open(sq_csv) as fh:
+        for r in csv.DictReader(fh):
+            if "hydra" in r["Kernel_Name"]:
+                tot[r["Kernel_Name"]][r["Counter_Name"]] += float(r["Counter_Value"])
+                disp[r["Kernel_Name"]].add(r["Dispatch_Id"])
+    print("| kernel | dispatches | waves | VALU insts | VMEM read insts | LDS insts | VALU active / wave cycles |")
+    print("|---|---|---|---|---|---|---|")
+    for k, c in tot.items():
+        n = len(disp[k])
+        name = k.split("(")[0].replace("void ", "")
+        act = c["SQ_ACTIVE_INST_VALU"] / max(c["SQ_WAVE_CYCLES"], 1.0)
+        print(f"| `{name}` | {n} | {c['SQ_WAVES'] / n:.0f} | {c['SQ_INSTS_VALU'] / n:.3e} | "
+              f"{c['SQ_INSTS_VMEM_RD'] / n:.3e} | {c['SQ_INSTS_LDS'] / n:.3e} | {act:.2f} |")
+
+
 def summarize_trace(db_path, top=20):
     """Per-kernel totals from a rocprofv3 --kernel-trace --stats database."""
     import sqlite3
@@ -245,6 +269,9 @@ def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-counters":
         summarize_counters(sys.argv[2], sys.argv[3], int(sys.argv[4]))
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "summarize-instructions":
+        summarize_instructions(sys.argv[2])
         return
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-trace":
         summarize_trace(sys.argv[2])
