@@ -75,6 +75,52 @@ def build_case(nx, ny, nz, seed=1, precision=0, overset=True):
     return m, p, cu, gp, donors
 
 
+def stream_copy_gbs(mib=1024, reps=10):
+    """Measured copy ceiling: a device-to-device copy of mib MiB (reads and
+    writes mib MiB each), timed with HIP events over reps copies, through the
+    HIP runtime the library already loaded.  GB/s of read + write bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
+    hip.hipFree.argtypes = [vp]
+    hip.hipMemset.argtypes = [vp, ctypes.c_int, sz]
+    hip.hipMemcpyAsync.argtypes = [vp, vp, sz, ctypes.c_int, vp]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+
+    def chk(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc})")
+    n = mib << 20
+    a, b, e0, e1 = vp(), vp(), vp(), vp()
+    chk(hip.hipMalloc(ctypes.byref(a), n), "hipMalloc")
+    try:
+        chk(hip.hipMalloc(ctypes.byref(b), n), "hipMalloc")
+        try:
+            chk(hip.hipMemset(a, 1, n), "hipMemset")
+            chk(hip.hipEventCreate(ctypes.byref(e0)), "hipEventCreate")
+            chk(hip.hipEventCreate(ctypes.byref(e1)), "hipEventCreate")
+            chk(hip.hipMemcpyAsync(b, a, n, 3, None), "hipMemcpyAsync")  # warm-up
+            chk(hip.hipEventRecord(e0, None), "hipEventRecord")
+            for _ in range(reps):
+                chk(hip.hipMemcpyAsync(b, a, n, 3, None), "hipMemcpyAsync")
+            chk(hip.hipEventRecord(e1, None), "hipEventRecord")
+            chk(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+            ms = ctypes.c_float()
+            chk(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1), "hipEventElapsedTime")
+            hip.hipEventDestroy(e0)
+            hip.hipEventDestroy(e1)
+        finally:
+            hip.hipFree(b)
+    finally:
+        hip.hipFree(a)
+    return 2.0 * n * reps / (ms.value * 1e-3) / 1e9
+
+
 def cpu_baseline(m, p, cu, gp, donors=None, seconds_target=20.0):
     """The serial C oracle (a port of the reference's FP64 path, 1 thread)
     on a bounded sample of the same workload: whole steps of the same case
@@ -364,6 +410,8 @@ def main():
                      "flux_avg_ms": flux_avg_ms, "bytes_per_cell": bpc},
     }
     s.close()
+    # measured copy ceiling beside the published peak
+    out["roofline"]["copy_ceiling"] = stream_copy_gbs()
     log = lambda msg: print(msg, file=sys.stderr, flush=True)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(log)
