@@ -396,6 +396,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == 0 else "f32-bulk/f64-state",
         "data": "synthetic",
+        "cell_steps_per_s": value / 4.0,  # whole 4-stage steps, comparable across RK variants
         "config": {"workload": (f"box {args.n[0]}x{args.n[1]}x{args.n[2]} hexes x6 tets cut at r=0.15 + "
                                 "annulus 10x112 hexes x6 around the r=0.1 wall, overset donor lists"
                                 if donors is not None else
