@@ -110,6 +110,17 @@ int hp_step(hp_ctx *ctx, int nsteps);
  * the driver's Velocity.dat probe |u_g(1)| (main.F90:640-642, :654). */
 int hp_probe_cell(hp_ctx *ctx, int cell, double *prims);
 
+/* Health of the case's current state.  No counterpart in the reference,
+ * which steps on through a non-finite state; for hosts (the request loop)
+ * that retire a case that has failed.  *status: 0 healthy; bit 0 (1) a
+ * non-finite conserved variable or primitive of an interior cell; bit 1 (2)
+ * a gas pressure <= 0 or a liquid pressure <= -pinf.  The primitives are
+ * those last evaluated (every step ends with an evaluation; after
+ * hp_upload_state there are none until the first).  Synchronises. */
+#define HP_HEALTH_NONFINITE 1
+#define HP_HEALTH_PRESSURE 2
+int hp_health(hp_ctx *ctx, int *status);
+
 int hp_synchronize(hp_ctx *ctx);
 const char *hp_last_error(const hp_ctx *ctx);
 
@@ -172,6 +183,10 @@ int hp_batch_step(hp_batch *b, int nsteps, int use_graph);
 int hp_batch_synchronize(hp_batch *b);
 /* elapsed milliseconds of the last hp_batch_step (HIP events) */
 int hp_batch_last_ms(hp_batch *b, double *ms);
+/* hp_health of every member at once: status[slot] as hp_health's, 0 for a
+ * free slot; n >= max_cases.  Runs after the batch's queued work and
+ * synchronises. */
+int hp_batch_health(hp_batch *b, int *status, int n);
 const char *hp_batch_last_error(const hp_batch *b);
 /* device bytes hp_create allocates for such a case (admission by budget) */
 size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision);

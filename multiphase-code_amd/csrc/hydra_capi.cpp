@@ -66,6 +66,7 @@ struct hp_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    int *health = nullptr;          // per slot, device
     std::string err;
 };
 
@@ -80,6 +81,7 @@ struct hp_ctx {
     hp_batch *batch = nullptr;  // the batch this case belongs to, if any
     Donors D{};                 // overset donor lists (n = 0: none)
     std::vector<void *> dbufs;  // their device buffers
+    int *health = nullptr;      // device flag word of hp_health
     std::string err;
     // timing of the last hp_step
     std::vector<hipEvent_t> fev;  // start/stop pairs around flux launches
@@ -320,6 +322,7 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     ALLOC(s.q, (size_t)Q_N * ntot) ALLOC(s.grad, (size_t)3 * G_N * ne) ALLOC(s.phi, ne)
     ALLOC(s.vis, 8 * (size_t)ne) ALLOC(s.d24, 10 * (size_t)ne) ALLOC(s.d2q, 10 * (size_t)ne)
     ALLOC(s.anu, 2 * (size_t)ne) ALLOC(s.rhs, 10 * (size_t)ne) ALLOC(s.scal, SC_N) ALLOC(s.itot, 1)
+    ALLOC(c->health, 1)
     if (params->precision == 1) {
         ALLOC(s.g32, (size_t)3 * G_N * ne) ALLOC(s.phi32, ne)
     } else {
@@ -624,6 +627,18 @@ int hp_last_timing(hp_ctx *c, double *total_ms, double *flux_ms, int *flux_launc
     return HP_OK;
 }
 
+int hp_health(hp_ctx *c, int *status)
+{
+    if (!c || !status) return fail(c, HP_ERR_ARG, "hp_health: bad argument");
+    if (int rc = sync_owner(c)) return rc;
+    HIPCHK(c, hipMemsetAsync(c->health, 0, sizeof(int), c->st));
+    launch_health(c->ts.view(), c->G, c->health, c->st);
+    if (int rc = check_launch(c, "health")) return rc;
+    HIPCHK(c, hipMemcpyAsync(status, c->health, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return HP_OK;
+}
+
 int hp_probe_cell(hp_ctx *c, int cell, double *prims)
 {
     if (!c || !prims || cell < 1 || cell > c->s.nt) return fail(c, HP_ERR_ARG, "hp_probe_cell: bad argument");
@@ -823,6 +838,11 @@ int hp_batch_create(hp_batch **out, int max_cases, int max_cells, int max_ghosts
         hp_batch_destroy(b);
         return bfail(nullptr, rc, msg);
     }
+    if (int rc = alloc(&b->health, (size_t)max_cases)) {
+        std::string msg = b->err;
+        hp_batch_destroy(b);
+        return bfail(nullptr, rc, msg);
+    }
     b->members.assign(max_cases, nullptr);
     batch_grids(b);
     *out = b;
@@ -936,6 +956,23 @@ int hp_batch_step(hp_batch *b, int nsteps, int use_graph)
     }
     BHIPCHK(b, hipEventRecord(b->e1, b->st));
     BHIPCHK(b, hipGetLastError());
+    return HP_OK;
+}
+
+int hp_batch_health(hp_batch *b, int *status, int n)
+{
+    const int slots = b ? (int)b->members.size() : 0;
+    if (!b || !status || n < slots) return bfail(b, HP_ERR_ARG, "hp_batch_health: need one int per case slot");
+    BHIPCHK(b, hipMemsetAsync(b->health, 0, sizeof(int) * slots, b->st));
+    if (b->ncases > 0) {
+        Grid G = b->G;
+        if (!b->grid_cell) G.cell = round8(b->ts.nct);
+        G.persist = G.cell < b->ts.nct ? 1 : 0;
+        launch_health(b->ts.view(), G, b->health, b->st);
+        BHIPCHK(b, hipGetLastError());
+    }
+    BHIPCHK(b, hipMemcpyAsync(status, b->health, sizeof(int) * slots, hipMemcpyDeviceToHost, b->st));
+    BHIPCHK(b, hipStreamSynchronize(b->st));
     return HP_OK;
 }
 

@@ -2008,6 +2008,35 @@ __global__ void __launch_bounds__(TILE) k_end_step(Batch B, int want_diffs)
     }
 }
 
+// Health of each case slot's interior cells: flags[slot] |= 1 for a
+// non-finite conserved variable or primitive, |= 2 for pg <= 0 or
+// pp + pinf <= 0 (the host clears flags first).  An OR of flags: the result
+// does not depend on the order the tiles run in.
+template <int MODE>
+__global__ void __launch_bounds__(TILE) k_health(Batch B, int *flags)
+{
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        const int slot = (MODE == 2) ? 0 : B.ct[t].cs;
+        int bad = 0;
+        if ((int)threadIdx.x < C.count) {
+            const State &s = C.S;
+            const int i = C.first + (int)threadIdx.x, ne = s.ne;
+            const size_t nt = s.nt;
+            for (int k = 0; k < 10; k++)
+                if (!isfinite(s.cn[(size_t)k * ne + i])) bad |= HP_HEALTH_NONFINITE;
+            for (int k = 0; k < Q_N; k++)
+                if (!isfinite(s.q[(size_t)k * nt + i])) bad |= HP_HEALTH_NONFINITE;
+            if (s.q[(size_t)Q_PG * nt + i] <= 0.0 || s.q[(size_t)Q_PP * nt + i] + C.P.pinf <= 0.0)
+                bad |= HP_HEALTH_PRESSURE;
+        }
+        const bool any1 = __any(bad & HP_HEALTH_NONFINITE), any2 = __any(bad & HP_HEALTH_PRESSURE);
+        if (threadIdx.x == 0 && (any1 || any2))
+            atomicOr(&flags[slot], (any1 ? HP_HEALTH_NONFINITE : 0) | (any2 ? HP_HEALTH_PRESSURE : 0));
+    }
+}
+
 // MODE of a launch: 2 for a context on its own, 1 for a persistent grid
 // (G.persist bit of the family), 0 otherwise
 #define PICK(bit, K, ...) \
@@ -2082,6 +2111,11 @@ void launch_rk_update(const Batch &B, const Grid &G, int stage, hipStream_t st)
 void launch_end_step(const Batch &B, const Grid &G, int want_diffs, hipStream_t st)
 {
     hipLaunchKernelGGL(PICK(1, k_end_step), dim3(G.cell), dim3(TILE), 0, st, B, want_diffs);
+}
+
+void launch_health(const Batch &B, const Grid &G, int *flags, hipStream_t st)
+{
+    hipLaunchKernelGGL(PICK(1, k_health), dim3(G.cell), dim3(TILE), 0, st, B, flags);
 }
 
 #undef PICK

@@ -101,6 +101,12 @@ module hydra_hip_mod
       integer(c_int), value :: cell
       real(c_double), intent(out) :: prims(14)
     end function
+    ! 0 healthy; bit 0 non-finite state, bit 1 non-positive pressure
+    integer(c_int) function hp_health(ctx, status) bind(C, name="hp_health")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: status
+    end function
     integer(c_int) function hp_synchronize(ctx) bind(C, name="hp_synchronize")
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx

@@ -48,6 +48,7 @@ class Batch:
         if rc != 0:
             raise HydraError(f"hp_batch_create failed ({rc}): {L.hp_batch_last_error(None).decode()}")
         self.h = h
+        self.max_cases = int(max_cases)
         self.members: dict[int, Solver] = {}
         self.grid = (0, 0, 0)
 
@@ -81,6 +82,13 @@ class Batch:
         v = ctypes.c_double()
         self._chk(self.L.hp_batch_last_ms(self.h, ctypes.byref(v)))
         return v.value
+
+    def health(self) -> dict[int, int]:
+        """hp_health status of every member, by slot (hp_batch_health)."""
+        n = self.max_cases
+        buf = (ctypes.c_int * n)()
+        self._chk(self.L.hp_batch_health(self.h, buf, n))
+        return {slot: buf[slot] for slot in self.members}
 
     def close(self):
         if getattr(self, "h", None):

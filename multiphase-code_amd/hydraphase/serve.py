@@ -9,15 +9,20 @@ and the number of steps to run.  The loop
 * advances all resident cases together, one hp_batch_step of at most
   `chunk` steps at a time (each case with its own dt), and
 * retires a case as soon as it has run its steps: its state is copied back
-  and its device memory freed, which lets waiting requests in.
+  and its device memory freed, which lets waiting requests in, and
+* after every chunk checks the health of every resident case (hp_batch_health:
+  a non-finite state or a non-positive pressure) and retires a failed case
+  at once, with its error recorded and no result, so that it holds neither
+  a slot nor HBM.
 
 Requests that arrive while others are running join at the next chunk
 boundary, so their work overlaps on the device with the cases already
 there.  A case's arithmetic reads only its own data, so its result is the
 same bit for bit whether it ran alone or in any mix.
 
-Reported: throughput in cell-stage updates/s over the makespan and the
-time-to-solution (arrival to result on the host) p50/p99.
+Reported: throughput in cell-stage updates/s over the makespan, the
+time-to-solution (arrival to result on the host) p50/p99 of the requests
+that completed, and the number that failed.
 """
 from __future__ import annotations
 
@@ -46,6 +51,7 @@ class Request:
     admit: float = math.nan
     finish: float = math.nan
     result: tuple = None           # (cu, gp, time, total) after the run
+    error: str = None              # set when the case failed (no result)
     extra: dict = field(default_factory=dict)
 
 
@@ -68,10 +74,12 @@ def poisson_requests(n: int, rate: float = 20.0, seed: int = 1, nmin: int = 10, 
 
 class Server:
     def __init__(self, budget_bytes: float = 200e9, max_cases: int = 256, chunk: int = 25,
-                 graph: bool = False, grid=(0, 0, 0), bytes_per_cell: float = 1000.0):
+                 graph: bool = False, grid=(0, 0, 0), bytes_per_cell: float = 1000.0,
+                 check_health: bool = True):
         self.budget = int(budget_bytes)
         self.chunk = int(chunk)
         self.graph = graph
+        self.check_health = check_health
         max_cells = int(self.budget / bytes_per_cell)
         self.batch = HB.Batch(max_cases, max_cells, max(1, max_cells // 4))
         self.batch.set_grid(*grid)
@@ -84,6 +92,8 @@ class Server:
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting: collections.deque[Request] = collections.deque()
         active: dict[int, tuple[Request, Solver, int]] = {}
+        slot_of: dict[int, int] = {}
+        failed = 0
         used = 0
         peak = 0
         updates = 0
@@ -101,7 +111,7 @@ class Server:
                 r = waiting.popleft()
                 s = Solver(r.mesh, r.params)
                 s.upload(r.cu, r.gp, 0.0, 0)
-                self.batch.add(s)
+                slot_of[r.rid] = self.batch.add(s)
                 active[r.rid] = (r, s, r.spec.steps)
                 used += r.nbytes
                 peak = max(peak, used)
@@ -116,26 +126,35 @@ class Server:
             self.batch.synchronize()
             gpu_ms += self.batch.last_ms()
             updates += 4 * k * cells
+            health = self.batch.health() if self.check_health else {}
             for rid in list(active):
                 r, s, rem = active[rid]
                 rem -= k
-                if rem > 0:
+                bad = health.get(slot_of[rid], 0)
+                if rem > 0 and not bad:
                     active[rid] = (r, s, rem)
                     continue
                 self.batch.remove(s)
-                r.result = s.download()
+                if bad:
+                    r.error = (f"state failed its health check (status {bad}) after "
+                               f"{r.spec.steps - rem} of {r.spec.steps} steps")
+                    failed += 1
+                else:
+                    r.result = s.download()
                 s.close()
                 del active[rid]
+                del slot_of[rid]
                 used -= r.nbytes
                 r.finish = clock()
                 if log:
-                    log(f"request {r.rid} (n={r.spec.n}, {r.spec.steps} steps) done at {r.finish:.2f} s")
+                    log(f"request {r.rid} (n={r.spec.n}, {r.spec.steps} steps) "
+                        f"{'failed' if bad else 'done'} at {r.finish:.2f} s")
         makespan = clock()
-        tts = np.array([r.finish - r.arrival for r in requests])
-        return {"requests": len(requests), "makespan_s": makespan, "updates": updates,
+        tts = np.array([r.finish - r.arrival for r in requests if r.error is None])
+        pct = (lambda q: float(np.percentile(tts, q))) if tts.size else (lambda q: math.nan)
+        return {"requests": len(requests), "failed": failed, "makespan_s": makespan, "updates": updates,
                 "updates_per_s": updates / makespan, "gpu_busy_s": gpu_ms * 1e-3,
-                "tts_p50_s": float(np.percentile(tts, 50)), "tts_p99_s": float(np.percentile(tts, 99)),
-                "peak_resident_bytes": peak}
+                "tts_p50_s": pct(50), "tts_p99_s": pct(99), "peak_resident_bytes": peak}
 
 
 def alone(r: Request, precision: int | None = None):

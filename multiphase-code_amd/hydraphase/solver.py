@@ -125,6 +125,13 @@ class Solver:
     def synchronize(self):
         self._chk(self.L.hp_synchronize(self.h))
 
+    def health(self) -> int:
+        """0 if the state is healthy, else HEALTH_NONFINITE | HEALTH_PRESSURE
+        bits (hp_health)."""
+        v = ctypes.c_int()
+        self._chk(self.L.hp_health(self.h, ctypes.byref(v)))
+        return v.value
+
     # -- overset -----------------------------------------------------------
     def set_donors(self, recv, donors, weights):
         """Fixed donor lists (0-based receptor ids (n,), donors (n, 4),
