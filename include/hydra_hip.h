@@ -191,6 +191,13 @@ const char *hp_batch_last_error(const hp_batch *b);
 /* device bytes hp_create allocates for such a case (admission by budget) */
 size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision);
 
+/* Self-check of the FP64 division the kernels share one reciprocal for (no
+ * counterpart in the reference; for tests and for hosts that validate a new
+ * device or toolchain): *mismatches = the number of pairs (a[k], b[k]) whose
+ * shared-divisor quotient differs in any bit from the hardware a[k] / b[k]
+ * (must be 0).  Allocates and synchronises; errors in hp_last_error(NULL). */
+int hp_check_division(const double *a, const double *b, size_t n, size_t *mismatches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -996,6 +996,32 @@ int hp_batch_last_ms(hp_batch *b, double *ms)
 
 const char *hp_batch_last_error(const hp_batch *b) { return b ? b->err.c_str() : g_create_err.c_str(); }
 
+int hp_check_division(const double *a, const double *b, size_t n, size_t *mismatches)
+{
+    if ((n && (!a || !b)) || !mismatches) return fail(nullptr, HP_ERR_ARG, "hp_check_division: bad argument");
+    *mismatches = 0;
+    if (!n) return HP_OK;
+    double *da = nullptr, *db = nullptr;
+    unsigned long long *dbad = nullptr, bad = 0;
+    hipError_t e = hipMalloc(&da, n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&db, n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dbad, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemcpy(da, a, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, b, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(dbad, 0, sizeof(unsigned long long));
+    if (e == hipSuccess) {
+        launch_check_division(da, db, n, dbad, nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&bad, dbad, sizeof(bad), hipMemcpyDeviceToHost);
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dbad);
+    if (e != hipSuccess) return fail(nullptr, HP_ERR_HIP, std::string("hp_check_division: ") + hipGetErrorString(e));
+    *mismatches = (size_t)bad;
+    return HP_OK;
+}
+
 size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision)
 {
     if (nodes < 0 || neles < 0 || nghosts < 0) return 0;

@@ -109,6 +109,7 @@ void launch_flux(const Batch &B, const Grid &G, int stage, hipStream_t st);
 void launch_rk_update(const Batch &B, const Grid &G, int stage, hipStream_t st);
 void launch_end_step(const Batch &B, const Grid &G, int want_diffs, hipStream_t st);
 void launch_health(const Batch &B, const Grid &G, int *flags, hipStream_t st);
+void launch_check_division(const double *a, const double *b, size_t n, unsigned long long *bad, hipStream_t st);
 void launch_dt_from_prims(const Batch &B, const Grid &G, hipStream_t st);
 
 // overset donor lists of one case: per receptor four donor ids and weights

@@ -2113,6 +2113,21 @@ void launch_end_step(const Batch &B, const Grid &G, int want_diffs, hipStream_t 
     hipLaunchKernelGGL(PICK(1, k_end_step), dim3(G.cell), dim3(TILE), 0, st, B, want_diffs);
 }
 
+// self-check of the shared-divisor division: pairs where divide() and the
+// hardware a / b differ in any bit (NaNs compare equal to NaNs)
+__global__ void k_check_division(const double *a, const double *b, size_t n, unsigned long long *bad)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        const double x = divide(a[k], divisor(b[k])), y = a[k] / b[k];
+        if (__double_as_longlong(x) != __double_as_longlong(y) && !(isnan(x) && isnan(y))) atomicAdd(bad, 1ull);
+    }
+}
+
+void launch_check_division(const double *a, const double *b, size_t n, unsigned long long *bad, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_check_division, dim3(1024), dim3(256), 0, st, a, b, n, bad);
+}
+
 void launch_health(const Batch &B, const Grid &G, int *flags, hipStream_t st)
 {
     hipLaunchKernelGGL(PICK(1, k_health), dim3(G.cell), dim3(TILE), 0, st, B, flags);

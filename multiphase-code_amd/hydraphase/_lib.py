@@ -44,7 +44,7 @@ _lib = None
 EXPORTS = ["hp_create", "hp_destroy", "hp_upload_state", "hp_download_state", "hp_primitive",
            "hp_dt_min", "hp_set_dt", "hp_smooth", "hp_viscous", "hp_flux", "hp_rk_stage", "hp_rk",
            "hp_end_step", "hp_step", "hp_synchronize", "hp_last_error", "hp_get_array",
-           "hp_set_array", "hp_last_timing", "hp_probe_cell", "hp_health", "hp_batch_health",
+           "hp_set_array", "hp_last_timing", "hp_probe_cell", "hp_health", "hp_batch_health", "hp_check_division",
            "hp_batch_create", "hp_batch_destroy", "hp_batch_add", "hp_batch_remove", "hp_batch_set_grid",
            "hp_batch_step", "hp_batch_synchronize", "hp_batch_last_ms", "hp_batch_last_error",
            "hp_case_bytes", "hp_set_donors", "hp_apply_donors"]
@@ -80,6 +80,7 @@ def lib():
     L.hp_probe_cell.argtypes = [vp, c_int, dp]
     L.hp_health.argtypes = [vp, ip]
     L.hp_batch_health.argtypes = [vp, ip, c_int]
+    L.hp_check_division.argtypes = [dp, dp, c_size, ctypes.POINTER(c_size)]
     L.hp_batch_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int]
     L.hp_batch_destroy.argtypes = [vp]
     L.hp_batch_add.argtypes = [vp, vp, ip]
