@@ -77,3 +77,27 @@ def test_null_arguments_rejected():
                        None, None, None) == 1
     assert L.hp_step(None, 1) == 1
     assert L.hp_upload_state(None, None, 0, None, 0.0, 0) == 1
+
+
+def test_bad_order_and_precision_rejected():
+    m = M.box_mesh(2, 2, 2)
+    p = M.water_air_params()
+    p["myorder"] = 3
+    rc, msg, h = _create(m, p)
+    assert rc == 1 and "myorder" in msg and not h.value
+    p = M.water_air_params()
+    p["precision"] = 2
+    rc, msg, h = _create(m, p)
+    assert rc == 1 and "precision" in msg and not h.value
+
+
+def test_health_and_division_check_arguments():
+    L = _lib.lib()
+    st = ctypes.c_int(7)
+    assert L.hp_health(None, ctypes.byref(st)) == 1
+    assert L.hp_batch_health(None, ctypes.byref(st), 1) == 1
+    n = ctypes.c_size_t(3)
+    # no pairs: nothing to check, no device work
+    assert L.hp_check_division(None, None, 0, ctypes.byref(n)) == 0 and n.value == 0
+    assert L.hp_check_division(None, None, 4, ctypes.byref(n)) == 1
+    assert "hp_check_division" in L.hp_last_error(None).decode()
