@@ -103,3 +103,40 @@ def test_served_mix_retires_failed_request():
         cu, gp, t, tot = r.result
         assert tot == r.spec.steps == ref[3] and t == ref[2]
         assert np.array_equal(cu, ref[0]) and np.array_equal(gp, ref[1]), r.rid
+
+
+def test_reference_breakdown_flagged():
+    """A water sphere (r = 0.2) in air on a 48^3 x 6 box: by step 10 the
+    reference's own step drives the gas internal energy of three nearly
+    gas-free cells negative, the mixture-pressure root takes the square root
+    of a negative number and those cells become NaN (the oracle gives the
+    same NaNs).  The health check flags that case, alone and in a batch,
+    and not the healthy case beside it."""
+    m = M.box_mesh(48, 48, 48, jitter=0.1, seed=1)
+    p = M.water_air_params(cfl=0.5, viscous_gas=True)
+    cu, gp = M.sphere_state(m, p, [(0.0, 0.1, 0.0, 0.2)])
+    s = Solver(m, p)
+    try:
+        s.upload(cu, gp, 0.0, 0)
+        s.step(9)
+        assert s.health() == 0
+        s.step(1)
+        assert s.health() & _lib.HEALTH_NONFINITE
+        assert np.isnan(s.download()[0]).any()
+    finally:
+        s.close()
+    mh, ph, cuh, gph = small()
+    b = HB.Batch(2, m.neles + mh.neles, m.nghosts + mh.nghosts)
+    sols = []
+    try:
+        for mm, pp, cc, gg in ((m, p, cu, gp), (mh, ph, cuh, gph)):
+            x = Solver(mm, pp)
+            x.upload(cc, gg, 0.0, 0)
+            sols.append((b.add(x), x))
+        b.step(10, graph=True)
+        h = b.health()
+        assert h[sols[0][0]] & _lib.HEALTH_NONFINITE and h[sols[1][0]] == 0
+    finally:
+        b.close()
+        for _, x in sols:
+            x.close()
