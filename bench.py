@@ -285,7 +285,8 @@ def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
             "updates_per_s": batched, "ms_per_step": res["ms"] / steps,
             "plain_fp64_one_case_updates_per_s": plain, "speedup_vs_plain_fp64": batched / plain,
             "tuning": {"morton": tuning["morton"], "grid": list(tuning["grid"])},
-            "mixture_rel_l2_vs_fp64_sample": errs, "sample_cases": sample}
+            "mixture_rel_l2_vs_fp64_sample": errs, "sample_cases": sample,
+            "cases_unhealthy_at_end": int(sum(1 for h in res["health"] if h))}
 
 
 def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
@@ -299,8 +300,9 @@ def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
     finally:
         srv.close()
     k = max(1, int(round(sample_frac * n_requests)))
+    done = [r for r in reqs if r.error is None]  # requests retired as failed carry no result
     errs = []
-    for r in reqs[:: max(1, n_requests // k)][:k]:
+    for r in done[:: max(1, len(done) // k)][:k]:
         ref = SV.alone(r, precision=0)
         errs.append(float(SV.mixture_rel_l2(r.result[0], ref[0]).max()))
     cells = sum(int(r.mesh.iblank.sum()) for r in reqs)
@@ -308,7 +310,7 @@ def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
                         "50..500 steps, 200 GB budget, FP32 bulk",
             "updates_per_s": st["updates_per_s"], "tts_p50_s": st["tts_p50_s"], "tts_p99_s": st["tts_p99_s"],
             "makespan_s": st["makespan_s"], "gpu_busy_s": st["gpu_busy_s"],
-            "peak_resident_gb": st["peak_resident_bytes"] / 1e9,
+            "peak_resident_gb": st["peak_resident_bytes"] / 1e9, "requests_failed": st["failed"],
             "mixture_rel_l2_vs_fp64_sample": errs}
 
 

@@ -227,33 +227,39 @@ def tune(specs, precision: int = 1, steps: int = 3, myflux: int = 2, log=None) -
 def run_ensemble(specs, precision: int = 1, tuning: dict | None = None, graph: bool = True,
                  myflux: int = 2):
     """Build the ensemble, advance every case by its own step count in one
-    batch (cases leave when done), and return per-case final states and the
-    timing: {"ms": total, "updates": cell-stage updates, "states": [...]}."""
+    batch (cases leave when done), and return per-case final states, their
+    hp_health status when they left, and the timing: {"ms": total,
+    "updates": cell-stage updates, "states": [...], "health": [...]}."""
     tuning = tuning or {"morton": False, "grid": (0, 0, 0)}
     cases = [build_case(s, precision, tuning["morton"], myflux) for s in specs]
     b = Batch(*capacity([c[0] for c in cases]))
     b.set_grid(*tuning["grid"])
-    sols = []
+    sols, slots = [], []
     for (m, p, cu, gp, _) in cases:
         s = Solver(m, p)
         s.upload(cu, gp, 0.0, 0)
-        b.add(s)
+        slots.append(b.add(s))
         sols.append(s)
     remaining = [s.steps for s in specs]
     active = [int(c[0].iblank.sum()) for c in cases]
     updates, total_ms, done = 0, 0.0, 0
     states = [None] * len(specs)
+    health = [0] * len(specs)
     t0 = time.perf_counter()
     while any(r > 0 for r in remaining):
         k = min(r for r in remaining if r > 0)
         b.step(k, graph=graph)
         total_ms += b.last_ms()
         updates += 4 * k * sum(int(a) for a, r in zip(active, remaining) if r > 0)
+        status = None
         for i, r in enumerate(remaining):
             if r <= 0:
                 continue
             remaining[i] = r - k
             if remaining[i] == 0:
+                if status is None:
+                    status = b.health()  # outside the timed steps
+                health[i] = status[slots[i]]
                 b.remove(sols[i])
                 states[i] = sols[i].download()
                 done += 1
@@ -262,5 +268,5 @@ def run_ensemble(specs, precision: int = 1, tuning: dict | None = None, graph: b
     perms = [c[4] for c in cases]
     for s in sols:
         s.close()
-    return {"ms": total_ms, "wall_s": wall, "updates": updates, "states": states, "meshes": [c[0] for c in cases],
-            "perms": perms}
+    return {"ms": total_ms, "wall_s": wall, "updates": updates, "states": states, "health": health,
+            "meshes": [c[0] for c in cases], "perms": perms}
