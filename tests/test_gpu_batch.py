@@ -149,3 +149,22 @@ def test_case_bytes_covers_allocation():
     s.close()
     print(f"case bytes: estimate {est / 2**20:.1f} MiB, allocated {used / 2**20:.1f} MiB")
     assert used <= 1.05 * est + (64 << 20), (used, est)
+
+
+def test_run_ensemble_matches_alone():
+    """hydraphase.batch.run_ensemble (the ensemble driver: Morton-ordered
+    cases, graph replay, each case leaving when its own step count is done)
+    returns per case the state of that case run alone, and its health."""
+    sp = [HB.CaseSpec(5, 0.3, 0.2, True, 21, steps=4), HB.CaseSpec(7, 0.5, 0.1, False, 22, steps=7),
+          HB.CaseSpec(4, 0.2, 0.15, True, 23, steps=2)]
+    res = HB.run_ensemble(sp, precision=0, tuning={"morton": True, "grid": (0, 0, 0)}, graph=True)
+    assert res["health"] == [0, 0, 0]
+    for k, s in enumerate(sp):
+        m, p, cu, gp, _ = HB.build_case(s, 0)
+        ref = alone((m, p, cu, gp, None), s.steps)
+        got = res["states"][k]
+        perm = res["perms"][k]
+        cu_got = np.empty_like(got[0])
+        cu_got[:, perm] = got[0]  # back to the generated cell order
+        assert got[3] == ref[3] == s.steps and got[2] == ref[2]
+        assert np.array_equal(cu_got, ref[0]) and np.array_equal(got[1], ref[1]), k
