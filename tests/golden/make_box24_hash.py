@@ -1,12 +1,12 @@
-"""Pin the oracle's 100-step field of the configuration-1 workload by hash.
+"""Pin the oracle's 100-step field of the 24^3 x 6 jittered-box workload by hash.
 
 24^3 hexes x 6 = 82,944 jittered tets (seed 1), water sphere R=0.2 at the
 centre of an air stream at Mach 0.3 (air sound speed), CFL 0.5, 100 steps.
-Writes tests/golden/config1_100.json: SHA-256 of the conserved field (10 x
+Writes tests/golden/box24_100steps.json: SHA-256 of the conserved field (10 x
 neles float64, C order), the physical time, and per-variable norms, at step
 100 and (under "step10") at step 10.
 
-usage: python tests/golden/make_config1_hash.py
+usage: python tests/golden/make_box24_hash.py
 """
 import hashlib
 import json
@@ -42,5 +42,5 @@ out = {"neles": m.neles, "steps": 100, "time": t, "total": tot,
        "step10": {"time": t10, "total": tot10,
                   "cu_sha256": hashlib.sha256(np.ascontiguousarray(cu10).tobytes()).hexdigest(),
                   "cu_l2": [float(v) for v in np.sqrt((cu10 ** 2).sum(axis=1))]}}
-json.dump(out, open(os.path.join(ROOT, "tests", "golden", "config1_100.json"), "w"), indent=1)
+json.dump(out, open(os.path.join(ROOT, "tests", "golden", "box24_100steps.json"), "w"), indent=1)
 print(json.dumps(out, indent=1))

@@ -214,14 +214,14 @@ def test_first_order_steps_bitwise(flux):
         assert_same(f"ghost prims fused={fused}", gp, gpr)
 
 
-def test_config1_100_steps():
-    """Configuration 1 workload: 24^3 hexes x 6 = 82,944 tets, jittered,
+def test_box24_100_steps():
+    """The 24^3 x 6 jittered-box workload: 82,944 tets,
     water sphere R=0.2 in an air stream at Mach 0.3, CFL 0.5, 100 steps.
     The oracle's 100-step field is pinned by its SHA-256
-    (tests/golden/config1_100.json, written by tests/golden/make_config1_hash.py);
+    (tests/golden/box24_100steps.json, written by tests/golden/make_box24_hash.py);
     if the GPU field's hash differs, the oracle runs live and the relative
     L2 per conserved variable must stay <= 1e-10."""
-    spec = json.load(open(os.path.join(HERE, "golden", "config1_100.json")))
+    spec = json.load(open(os.path.join(HERE, "golden", "box24_100steps.json")))
     m = M.box_mesh(24, 24, 24, jitter=0.1, seed=1)
     p = M.water_air_params(cfl=0.5)
     cu0, gp0 = M.sphere_state(m, p, [(0.0, 0.0, 0.0, 0.2)])
@@ -231,7 +231,7 @@ def test_config1_100_steps():
     cu, gp, t, tot = s.download()
     assert tot == 100
     h = hashlib.sha256(np.ascontiguousarray(cu).tobytes()).hexdigest()
-    print(f"config1 100 steps: gpu sha256 {h[:16]} oracle {spec['cu_sha256'][:16]} time {t!r}")
+    print(f"box24 100 steps: gpu sha256 {h[:16]} oracle {spec['cu_sha256'][:16]} time {t!r}")
     if h == spec["cu_sha256"]:
         assert t == spec["time"]
         return

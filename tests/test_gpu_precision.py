@@ -1,9 +1,9 @@
 """FP32-bulk variant (hp_params.precision = 1) against the FP64 oracle.
 
-Configuration 3: the configuration-1 mesh, state and steps (24^3 hexes x 6 =
-82,944 jittered tets, water sphere R=0.2 in air at Mach 0.3, CFL 0.5).  The
+Workload: the 24^3 x 6 jittered box (82,944 tets, water sphere R=0.2 in
+air at Mach 0.3, CFL 0.5), 100 steps.  The
 FP64 GPU path is first tied to the oracle by the oracle's own hashes at
-steps 10 and 100 (tests/golden/config1_100.json); the FP32-bulk run is then
+steps 10 and 100 (tests/golden/box24_100steps.json); the FP32-bulk run is then
 compared with it per conserved variable: relative L2 and relative max-norm
 (both relative to that variable's FP64 norm), and for the mixture mass,
 momentum and energy (sums over the two phases).
@@ -37,7 +37,7 @@ VARS = ["phig*rog", "phig*rog*ug", "phig*rog*vg", "phig*rog*wg", "phig*rog*Eg",
 MIX = ["mass", "mom x", "mom y", "mom z", "energy"]
 
 
-def config3(myflux=2):
+def box24_case(myflux=2):
     m = M.box_mesh(24, 24, 24, jitter=0.1, seed=1)
     p = M.water_air_params(cfl=0.5, myflux=myflux)
     cu0, gp0 = M.sphere_state(m, p, [(0.0, 0.0, 0.0, 0.2)])
@@ -67,7 +67,7 @@ def errors(cu, ref):
 
 
 def compare(myflux, checkpoints, spec=None):
-    m, p, cu0, gp0 = config3(myflux)
+    m, p, cu0, gp0 = box24_case(myflux)
     ref = run(m, p, cu0, gp0, 0, checkpoints)
     if spec is not None:
         for n, h in ((10, spec["step10"]["cu_sha256"]), (100, spec["cu_sha256"])):
@@ -93,7 +93,7 @@ def compare(myflux, checkpoints, spec=None):
 
 
 def test_fp32_bulk_error_vs_fp64_oracle():
-    spec = json.load(open(os.path.join(HERE, "golden", "config1_100.json")))
+    spec = json.load(open(os.path.join(HERE, "golden", "box24_100steps.json")))
     rows = compare(2, [10, 100], spec)
     for n, (e32, efl) in rows.items():
         assert e32[0].max() <= FLOOR_FACTOR * efl[0].max(), (n, e32[0].max(), efl[0].max())
