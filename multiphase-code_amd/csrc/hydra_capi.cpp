@@ -216,8 +216,7 @@ void enqueue_step(const Batch &B, const Grid &G, hipStream_t st, hipEvent_t *fev
         if (fev) (void)hipEventRecord(fev[2 * (k - 1) + 1], st);
         if (D) launch_donor_gather(B.S1, *D, st);  // overset receptors of a lone case
     }
-    launch_eos(B, G, 0, st);       // main.F90:595
-    launch_end_step(B, G, 0, st);  // :619-625
+    launch_eos_end(B, G, st);      // main.F90:595, and cu = cn (:625); no monitor diffs here
 }
 
 }  // namespace

@@ -108,6 +108,7 @@ void launch_grad(const Batch &B, const Grid &G, hipStream_t st);
 void launch_flux(const Batch &B, const Grid &G, int stage, hipStream_t st);
 void launch_rk_update(const Batch &B, const Grid &G, int stage, hipStream_t st);
 void launch_end_step(const Batch &B, const Grid &G, int want_diffs, hipStream_t st);
+void launch_eos_end(const Batch &B, const Grid &G, hipStream_t st);  // primitive, then cu = cn
 void launch_health(const Batch &B, const Grid &G, int *flags, hipStream_t st);
 void launch_check_division(const double *a, const double *b, size_t n, unsigned long long *bad, hipStream_t st);
 void launch_dt_from_prims(const Batch &B, const Grid &G, hipStream_t st);

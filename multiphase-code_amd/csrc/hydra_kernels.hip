@@ -219,8 +219,10 @@ __global__ void k_geom_ghost(Mesh m, double xmax, double ymax, double zmax, doub
 // sr_primitiveplusbc.F90:36-172).  Optionally the CFL bound of
 // main.F90:541-545 reduced to one minimum per block with wave shuffles.
 // ---------------------------------------------------------------------------
+// cu (may be null): also store the rewritten conserved variables there, the
+// end-of-step cu = cn (main.F90:625) fused into the last primitive
 __device__ __forceinline__ void eos_cell(const Params &P, double *cn, int ne, int i, double *q, int nt,
-                                         double *dtloc, const double *dl)
+                                         double *dtloc, const double *dl, double *cu = nullptr)
 {
     const double c1 = cn[i], c2 = cn[(size_t)ne + i], c3 = cn[(size_t)2 * ne + i], c4 = cn[(size_t)3 * ne + i],
                  c5 = cn[(size_t)4 * ne + i], c6 = cn[(size_t)5 * ne + i], c7 = cn[(size_t)6 * ne + i],
@@ -277,16 +279,10 @@ __device__ __forceinline__ void eos_cell(const Params &P, double *cn, int ne, in
     QV(Q_TG, i) = tg; QV(Q_PHIG, i) = phig; QV(Q_UP, i) = up; QV(Q_VP, i) = vp; QV(Q_WP, i) = wp;
     QV(Q_PHIP, i) = phip; QV(Q_TP, i) = tp; QV(Q_ROP, i) = rop; QV(Q_ROG, i) = rog;
     const double mg = phig * rog, mp = phip * rop;
-    cn[i] = mg;
-    cn[(size_t)ne + i] = mg * ug;
-    cn[(size_t)2 * ne + i] = mg * vg;
-    cn[(size_t)3 * ne + i] = mg * wg;
-    cn[(size_t)4 * ne + i] = mg * egt;
-    cn[(size_t)5 * ne + i] = mp;
-    cn[(size_t)6 * ne + i] = mp * up;
-    cn[(size_t)7 * ne + i] = mp * vp;
-    cn[(size_t)8 * ne + i] = mp * wp;
-    cn[(size_t)9 * ne + i] = mp * ept;
+    const double nv[10] = {mg, mg * ug, mg * vg, mg * wg, mg * egt, mp, mp * up, mp * vp, mp * wp, mp * ept};
+    for (int k = 0; k < 10; k++) cn[(size_t)k * ne + i] = nv[k];
+    if (cu)
+        for (int k = 0; k < 10; k++) cu[(size_t)k * ne + i] = nv[k];
     if (dtloc) {
         // main.F90:541-545
         double ag = sqrt((P.gamag * pg) / rog);
@@ -1822,17 +1818,19 @@ __device__ __forceinline__ double wave_max(double v)
 
 // primitive (sr_primitive.F90:18-178) on interior cells; want_dt: also the
 // CFL bound per tile (main.F90:541-545)
-template <int MODE, bool WANT_DT>
+// KIND 0: primitives; 1: and the CFL bound per tile; 2: and cu = cn
+template <int MODE, int KIND>
 __global__ void __launch_bounds__(TILE) k_eos(Batch B)
 {
-    const bool want_dt = WANT_DT;
+    const bool want_dt = KIND == 1;
     TILE_LOOP(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
         const State &s = C.S;
         double loc = 1.0e8;
         if ((int)threadIdx.x < C.count)
-            eos_cell(C.P, s.cn, s.ne, C.first + (int)threadIdx.x, s.q, s.nt, want_dt ? &loc : nullptr, s.dl);
+            eos_cell(C.P, s.cn, s.ne, C.first + (int)threadIdx.x, s.q, s.nt, want_dt ? &loc : nullptr, s.dl,
+                     KIND == 2 ? s.cu : nullptr);
         if (want_dt) {
             const double v = wave_min(loc);
             if (threadIdx.x == 0) B.part[(size_t)4 * t] = v;
@@ -2045,9 +2043,14 @@ __global__ void __launch_bounds__(TILE) k_health(Batch B, int *flags)
 void launch_eos(const Batch &B, const Grid &G, int want_dt, hipStream_t st)
 {
     if (want_dt)
-        hipLaunchKernelGGL(PICK(1, k_eos, true), dim3(G.cell), dim3(TILE), 0, st, B);
+        hipLaunchKernelGGL(PICK(1, k_eos, 1), dim3(G.cell), dim3(TILE), 0, st, B);
     else
-        hipLaunchKernelGGL(PICK(1, k_eos, false), dim3(G.cell), dim3(TILE), 0, st, B);
+        hipLaunchKernelGGL(PICK(1, k_eos, 0), dim3(G.cell), dim3(TILE), 0, st, B);
+}
+
+void launch_eos_end(const Batch &B, const Grid &G, hipStream_t st)
+{
+    hipLaunchKernelGGL(PICK(1, k_eos, 2), dim3(G.cell), dim3(TILE), 0, st, B);
 }
 
 void launch_dt_final(const Batch &B, const Grid &G, hipStream_t st)
