@@ -66,6 +66,22 @@ def test_batch_equals_alone(precision):
             assert same(got[k], ref[k]), (precision, graph, k)
 
 
+@pytest.mark.parametrize("myflux,myorder", [(1, 2), (2, 1), (1, 1)])
+def test_batch_equals_alone_other_schemes(myflux, myorder):
+    """HLLC-Zien and first-order faces in a batch: each case equals its
+    alone run, FP64 and FP32 bulk."""
+    for precision in (0, 1):
+        cases = []
+        for s in specs()[:3]:
+            m, p, cu, gp, perm = HB.build_case(s, precision, myflux=myflux)
+            p["myorder"] = myorder
+            cases.append((m, p, cu, gp, perm))
+        ref = [alone(c, 3) for c in cases]
+        got = in_batch(cases, 3, graph=True)
+        for k in range(len(cases)):
+            assert same(got[k], ref[k]), (precision, k)
+
+
 def test_batch_fp64_case_matches_oracle():
     cases = [HB.build_case(s, 0) for s in specs()[:2]]
     got = in_batch(cases, 3)

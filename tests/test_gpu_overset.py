@@ -37,8 +37,10 @@ def test_gather_matches_formula(case):
     s.close()
 
 
-def test_overset_steps_match_oracle(case):
+@pytest.mark.parametrize("flux", [2, 1])
+def test_overset_steps_match_oracle(case, flux):
     m, p, cu, gp, recv, donors, w = case
+    p = dict(p, myflux=flux)
     s = Solver(m, p)
     s.upload(cu, gp, 0.0, 0)
     s.set_donors(recv, donors, w)
