@@ -113,6 +113,46 @@ __device__ __forceinline__ double quad_sum4(double v, int qbase)
     return t0 + t1 + t2 + t3;
 }
 
+// Lane J of this lane's quad by a DPP quad permutation (a VALU move; no
+// memory or LDS traffic).  Callers run with all four lanes of the quad active.
+template <int J>
+__device__ __forceinline__ int quad_lane32(int v)
+{
+    static_assert(J >= 0 && J < 4, "quad lane");
+    return __builtin_amdgcn_update_dpp(0, v, J * 0x55, 0xf, 0xf, false);
+}
+template <int J>
+__device__ __forceinline__ double quad_lane(double v)
+{
+    return __hiloint2double(quad_lane32<J>(__double2hiint(v)), quad_lane32<J>(__double2loint(v)));
+}
+template <int J>
+__device__ __forceinline__ float quad_lane(float v)
+{
+    return __int_as_float(quad_lane32<J>(__float_as_int(v)));
+}
+
+// Planes 0..NP-1 of one cell shared by the four lanes of its quad: lane f
+// loads planes f, f+4, f+8, ... (the last clamped to NP-1) into v, so the
+// quad issues a quarter of the loads; quad_plane<P> hands plane P to every
+// lane of the quad.
+template <int NP, typename T>
+__device__ __forceinline__ void quad_load(const T *a, size_t stride, int i, int f, T (&v)[(NP + 3) / 4])
+{
+#pragma unroll
+    for (int j = 0; j < (NP + 3) / 4; j++) {
+        int p = 4 * j + f;
+        if (p > NP - 1) p = NP - 1;
+        v[j] = a[(size_t)p * stride + i];
+    }
+}
+template <int P, typename T, int N>
+__device__ __forceinline__ T quad_plane(const T (&v)[N])
+{
+    static_assert(P >= 0 && (P >> 2) < N, "plane");
+    return quad_lane<P & 3>(v[P >> 2]);
+}
+
 // ---------------------------------------------------------------------------
 // geometry (sr_geometry.F90:147-289), once per mesh
 // ---------------------------------------------------------------------------
@@ -498,14 +538,30 @@ __device__ __forceinline__ void smooth2_cell(const Params &P, const State &s, co
 // plus the cell (boundary face) or plus the neighbour's 4 neighbours
 // (interior face); Newtonian stress and Fourier heat flux of both phases.
 // ---------------------------------------------------------------------------
-// Face-centred LSQ gradients of the eight face variables over the NM-cell
-// stencil mm (NM = 8 at an interior face: both cells' neighbours; NM = 5 at
-// a boundary face: the cell's neighbours and the cell) and the two phases'
-// stress and heat-flux contributions to the face (sr_viscousgp.F90:450-804).
-// NM is a constant so every stencil loop unrolls into straight-line code.
-template <int NM>
+// quad_lane with a lane index that is a constant once the caller's loop is
+// unrolled
+__device__ __forceinline__ double quad_from(double v, int k)
+{
+    switch (k) {
+    case 0: return quad_lane<0>(v);
+    case 1: return quad_lane<1>(v);
+    case 2: return quad_lane<2>(v);
+    default: return quad_lane<3>(v);
+    }
+}
+
+// Face-centred LSQ gradients of the eight face variables and the two
+// phases' stress and heat-flux contributions to face f (sr_viscousgp.F90:
+// 450-804).  The stencil of an interior face is the four face neighbours of
+// cell i, then the four of the cell across face f (m2); that of a boundary
+// face is cell i's neighbours and cell i itself (m2[0] = i).  Points 0-3 are
+// the same for the four faces of the quad: lane k holds point k's centroid
+// (xn, yn, zn) and primitives (qn) and hands them over by DPP, so the four
+// lanes run the stencil together, and a boundary-face lane drops points 5-7
+// by selects, leaving every sum the five-point one.
 __device__ __forceinline__ void viscous_stencil(const Params &P, const double *q, size_t nt, const Mesh &m,
-                                                const int (&mm)[8], double xf, double yf, double zf,
+                                                const int (&m2)[4], bool ghost, double xn, double yn, double zn,
+                                                const double (&qn)[8], double xf, double yf, double zf,
                                                 double scx, double scy, double scz, const double (&fv)[8],
                                                 double phigf, double phipf, double (&acc)[8])
 {
@@ -515,23 +571,28 @@ __device__ __forceinline__ void viscous_stencil(const Params &P, const double *q
     const double akg = amug * P.cpg / P.prg;
     const double amup = 0.001;
     const double akp = amup * P.cpp / P.prp;
-    double dx[NM], dy[NM], dz[NM];
+    double dx[8], dy[8], dz[8];
 #pragma unroll
-    for (int k = 0; k < NM; k++) {
-        dx[k] = m.xc[mm[k]] - xf;
-        dy[k] = m.yc[mm[k]] - yf;
-        dz[k] = m.zc[mm[k]] - zf;
+    for (int k = 0; k < 4; k++) {
+        dx[k] = quad_from(xn, k) - xf;
+        dy[k] = quad_from(yn, k) - yf;
+        dz[k] = quad_from(zn, k) - zf;
+    }
+#pragma unroll
+    for (int k = 4; k < 8; k++) {
+        dx[k] = m.xc[m2[k - 4]] - xf;
+        dy[k] = m.yc[m2[k - 4]] - yf;
+        dz[k] = m.zc[m2[k - 4]] - zf;
     }
     double a11 = dx[0] * dx[0], a22 = dy[0] * dy[0], a33 = dz[0] * dz[0];
     double a12 = dx[0] * dy[0], a13 = dx[0] * dz[0], a23 = dy[0] * dz[0];
 #pragma unroll
-    for (int k = 1; k < NM; k++) {
-        a11 = a11 + dx[k] * dx[k];
-        a22 = a22 + dy[k] * dy[k];
-        a33 = a33 + dz[k] * dz[k];
-        a12 = a12 + dx[k] * dy[k];
-        a13 = a13 + dx[k] * dz[k];
-        a23 = a23 + dy[k] * dz[k];
+    for (int k = 1; k < 8; k++) {
+        const bool use = !(ghost && k >= 5);
+        const double n11 = a11 + dx[k] * dx[k], n22 = a22 + dy[k] * dy[k], n33 = a33 + dz[k] * dz[k];
+        const double n12 = a12 + dx[k] * dy[k], n13 = a13 + dx[k] * dz[k], n23 = a23 + dy[k] * dz[k];
+        a11 = use ? n11 : a11; a22 = use ? n22 : a22; a33 = use ? n33 : a33;
+        a12 = use ? n12 : a12; a13 = use ? n13 : a13; a23 = use ? n23 : a23;
     }
     const double dnr = det3(a11, a12, a13, a12, a22, a23, a13, a23, a33);
 #pragma unroll
@@ -540,14 +601,15 @@ __device__ __forceinline__ void viscous_stencil(const Params &P, const double *q
         double gr[4][3];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            double dv = QV(slot[o + v], mm[0]) - fv[o + v];
+            double dv = quad_from(qn[o + v], 0) - fv[o + v];
             double b1 = dv * dx[0], b2 = dv * dy[0], b3 = dv * dz[0];
 #pragma unroll
-            for (int k = 1; k < NM; k++) {
-                dv = QV(slot[o + v], mm[k]) - fv[o + v];
-                b1 = b1 + dv * dx[k];
-                b2 = b2 + dv * dy[k];
-                b3 = b3 + dv * dz[k];
+            for (int k = 1; k < 8; k++) {
+                const bool use = !(ghost && k >= 5);
+                const double qk = k < 4 ? quad_from(qn[o + v], k) : QV(slot[o + v], m2[k - 4]);
+                dv = qk - fv[o + v];
+                const double n1 = b1 + dv * dx[k], n2 = b2 + dv * dy[k], n3 = b3 + dv * dz[k];
+                b1 = use ? n1 : b1; b2 = use ? n2 : b2; b3 = use ? n3 : b3;
             }
             gr[v][0] = det3(b1, b2, b3, a12, a22, a23, a13, a23, a33) / dnr;
             gr[v][1] = det3(a11, a12, a13, b1, b2, b3, a13, a23, a33) / dnr;
@@ -595,21 +657,29 @@ __device__ __forceinline__ void viscous_face(const Params &P, const State &s, co
                      scz = m.sc[(size_t)(3 * f + 2) * ne + i];
         const double xf = m.fc3[(size_t)(3 * f) * ne + i], yf = m.fc3[(size_t)(3 * f + 1) * ne + i],
                      zf = m.fc3[(size_t)(3 * f + 2) * ne + i];
+        // lane f's neighbour is point f of every face stencil of the quad
         const int neigh = m.nc[(size_t)f * ne + i];
+        double qn[8];
+        for (int v = 0; v < 8; v++) qn[v] = QV(slot[v], neigh);
+        const double xn = m.xc[neigh], yn = m.yc[neigh], zn = m.zc[neigh];
+        double vq[(Q_N + 3) / 4];
+        quad_load<Q_N>(q, nt, i, f, vq);
         double fv[8];
-        for (int v = 0; v < 8; v++) fv[v] = (QV(slot[v], i) + QV(slot[v], neigh)) * 0.5;
-        const double phigf = (QV(Q_PHIG, i) + QV(Q_PHIG, neigh)) * 0.5;
-        const double phipf = (QV(Q_PHIP, i) + QV(Q_PHIP, neigh)) * 0.5;
-        int mm[8];
-        for (int k = 0; k < 4; k++) mm[k] = m.nc[(size_t)k * ne + i];
-        if (neigh >= ne) {
-            mm[4] = i;
-            mm[5] = mm[6] = mm[7] = i;
-            viscous_stencil<5>(P, q, nt, m, mm, xf, yf, zf, scx, scy, scz, fv, phigf, phipf, acc);
-        } else {
-            for (int k = 0; k < 4; k++) mm[4 + k] = m.nc[(size_t)k * ne + neigh];
-            viscous_stencil<8>(P, q, nt, m, mm, xf, yf, zf, scx, scy, scz, fv, phigf, phipf, acc);
-        }
+        fv[0] = (quad_plane<Q_UG>(vq) + qn[0]) * 0.5;
+        fv[1] = (quad_plane<Q_VG>(vq) + qn[1]) * 0.5;
+        fv[2] = (quad_plane<Q_WG>(vq) + qn[2]) * 0.5;
+        fv[3] = (quad_plane<Q_TG>(vq) + qn[3]) * 0.5;
+        fv[4] = (quad_plane<Q_UP>(vq) + qn[4]) * 0.5;
+        fv[5] = (quad_plane<Q_VP>(vq) + qn[5]) * 0.5;
+        fv[6] = (quad_plane<Q_WP>(vq) + qn[6]) * 0.5;
+        fv[7] = (quad_plane<Q_TP>(vq) + qn[7]) * 0.5;
+        const double phigf = (quad_plane<Q_PHIG>(vq) + QV(Q_PHIG, neigh)) * 0.5;
+        const double phipf = (quad_plane<Q_PHIP>(vq) + QV(Q_PHIP, neigh)) * 0.5;
+        const bool ghost = neigh >= ne;
+        const int far = ghost ? i : neigh;
+        int m2[4];
+        for (int k = 0; k < 4; k++) m2[k] = ghost ? i : m.nc[(size_t)k * ne + far];
+        viscous_stencil(P, q, nt, m, m2, ghost, xn, yn, zn, qn, xf, yf, zf, scx, scy, scz, fv, phigf, phipf, acc);
     }
     for (int v = 0; v < 8; v++) {
         const double t0 = quad_get(acc[v], qbase, 0), t1 = quad_get(acc[v], qbase, 1),
@@ -1021,15 +1091,6 @@ __device__ __forceinline__ void hllc_zien(const Params &P, const FState &L, cons
 
 #define GRAD(s, d, c) s_grad[(size_t)(3 * (s) + (d)) * ne + (c)]
 
-// One cell per quad of lanes, one face per lane (sr_hllcbatten.F90:77-555):
-// each lane reconstructs, clips and solves the Riemann problem of its own
-// face; the quad then sums the four face fluxes in face order through lane
-// shuffles, so every residual is the reference's (((0+F1)+F2)+F3)+F4 without
-// atomics.  The interphase source (sr_source.F90:46-155) is formed by the
-// quad (each lane contributes its neighbour's least-squares terms) and each
-// lane finishes the conserved variables k = f, f+4, f+8.
-// stage >= 1: apply the RK stage update (sr_RK.F90) from the fused residual;
-// stage == 0: store the residual in rhs (per-routine checks).
 // the residual r of element o (variable-major offset): stored in rhs
 // (stage 0, per-routine checks) or applied as RK stage `stage`
 // (sr_RK.F90:29-161); FP64 in every precision
@@ -1084,6 +1145,13 @@ __device__ __forceinline__ void source_and_update(const State &s, const Mesh &m,
     }
 }
 
+// One cell per quad of lanes, one face per lane (sr_hllcbatten.F90:77-555):
+// each lane reconstructs, clips and solves the Riemann problem of its own
+// face; the quad then sums the four face fluxes in face order through lane
+// shuffles, so every residual is the reference's (((0+F1)+F2)+F3)+F4 without
+// atomics, and lane f finishes the conserved variables k = f, f+4, f+8.
+// Cell i's primitives and gradients are loaded once per quad (quad_load) and
+// handed to the four lanes by DPP.
 template <int FLUX>
 __device__ __forceinline__ void flux_face(const Params &P, const State &s, const Mesh &m, int i, int f, int qbase,
                                           int stage)
@@ -1106,17 +1174,39 @@ __device__ __forceinline__ void flux_face(const Params &P, const State &s, const
         const double enx = scx / area, eny = scy / area, enz = scz / area;
         const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
         FState L, R;
-        load_fstate(q, nt, i, L);
+        {
+            // cell i's primitives: shared by the quad
+            double v[(Q_N + 3) / 4];
+            quad_load<Q_N>(q, nt, i, f, v);
+            L.rog = quad_plane<Q_ROG>(v); L.ug = quad_plane<Q_UG>(v); L.vg = quad_plane<Q_VG>(v);
+            L.wg = quad_plane<Q_WG>(v); L.tg = quad_plane<Q_TG>(v); L.phig = quad_plane<Q_PHIG>(v);
+            L.rop = quad_plane<Q_ROP>(v); L.up = quad_plane<Q_UP>(v); L.vp = quad_plane<Q_VP>(v);
+            L.wp = quad_plane<Q_WP>(v); L.tp = quad_plane<Q_TP>(v); L.pg = quad_plane<Q_PG>(v);
+            L.pp = quad_plane<Q_PP>(v);
+        }
         load_fstate(q, nt, neigh, R);
-        if (P.myorder == 2 && neigh < ne) {
+        if (P.myorder == 2) {
             // MUSCL reconstruction with one limiter per face pair
-            // (sr_hllcbatten.F90:166-224)
-            const double phimin = dmin(s.phi[i], s.phi[neigh]);
+            // (sr_hllcbatten.F90:166-224), for faces with a cell on the far
+            // side.  The quad shares cell i's gradients, so the four lanes
+            // stay together here: a face with a ghost on the far side forms
+            // the terms from cell i's own data and discards them.
+            const bool rec = neigh < ne;
+            const int nb = rec ? neigh : i;
+            double g[(3 * G_N + 3) / 4];
+            quad_load<3 * G_N>(s_grad, (size_t)ne, i, f, g);
+            const double phimin = dmin(s.phi[i], s.phi[nb]);
             const double xf0 = xf - xi, yf0 = yf - yi, zf0 = zf - zi;
-            const double xfn = xf - m.xc[neigh], yfn = yf - m.yc[neigh], zfn = zf - m.zc[neigh];
+            const double xfn = xf - m.xc[nb], yfn = yf - m.yc[nb], zfn = zf - m.zc[nb];
 #define REC(fld, sl)                                                                                  \
-    L.fld = L.fld + phimin * (xf0 * GRAD(sl, 0, i) + yf0 * GRAD(sl, 1, i) + zf0 * GRAD(sl, 2, i));     \
-    R.fld = R.fld + phimin * (xfn * GRAD(sl, 0, neigh) + yfn * GRAD(sl, 1, neigh) + zfn * GRAD(sl, 2, neigh));
+    {                                                                                                 \
+        const double lv = L.fld + phimin * (xf0 * quad_plane<3 * (sl)>(g) +                           \
+                                            yf0 * quad_plane<3 * (sl) + 1>(g) +                       \
+                                            zf0 * quad_plane<3 * (sl) + 2>(g));                       \
+        const double rv = R.fld + phimin * (xfn * GRAD(sl, 0, nb) + yfn * GRAD(sl, 1, nb) +           \
+                                            zfn * GRAD(sl, 2, nb));                                   \
+        if (rec) { L.fld = lv; R.fld = rv; }                                                          \
+    }
             REC(phig, G_PHIG) REC(ug, G_UG) REC(vg, G_VG) REC(wg, G_WG) REC(tg, G_TG) REC(pg, G_PG)
             REC(up, G_UP) REC(vp, G_VP) REC(wp, G_WP) REC(tp, G_TP) REC(pp, G_PP)
 #undef REC
@@ -1694,13 +1784,16 @@ __device__ __forceinline__ void flux_face_f32(const Params &P, const State &s, c
     source_and_update_f32(P, s, m, i, f, qbase, neigh, rhs, stage);
 }
 
-// viscous_stencil in FP32: differences of FP64 fields are formed in FP64
-// and rounded once; sums, determinants and stresses run in FP32.
-template <int NM>
+// viscous_stencil in FP32, with the quad's shared points as there:
+// differences of FP64 fields are formed in FP64 and rounded once; sums,
+// determinants and stresses run in FP32.  A boundary-face lane zeroes the
+// offsets of points 5-7, so they add nothing.
 __device__ __forceinline__ void viscous_stencil_f32(const Params &P, const double *q, size_t nt, const Mesh &m,
-                                                    const int (&mm)[8], double xf, double yf, double zf,
-                                                    float scx, float scy, float scz, const double (&fv)[8],
-                                                    float phigf, float phipf, float (&acc)[8])
+                                                    const int (&m2)[4], bool ghost, double xn, double yn,
+                                                    double zn, const double (&qn)[8], double xf, double yf,
+                                                    double zf, float scx, float scy, float scz,
+                                                    const double (&fv)[8], float phigf, float phipf,
+                                                    float (&acc)[8])
 {
 #pragma clang fp contract(fast)
     const int slot[8] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP};
@@ -1710,13 +1803,17 @@ __device__ __forceinline__ void viscous_stencil_f32(const Params &P, const doubl
     const float amup = 0.001f;
     const float akp = amup * (float)(P.cpp / P.prp);
     const float c23 = (float)P.c23;
-    float dx[NM], dy[NM], dz[NM];
+    float dx[8], dy[8], dz[8];
     float a11 = 0.0f, a22 = 0.0f, a33 = 0.0f, a12 = 0.0f, a13 = 0.0f, a23 = 0.0f;
 #pragma unroll
-    for (int k = 0; k < NM; k++) {
-        dx[k] = (float)(m.xc[mm[k]] - xf);
-        dy[k] = (float)(m.yc[mm[k]] - yf);
-        dz[k] = (float)(m.zc[mm[k]] - zf);
+    for (int k = 0; k < 8; k++) {
+        const bool use = !(ghost && k >= 5);
+        const double px = k < 4 ? quad_from(xn, k) : m.xc[m2[k - 4]];
+        const double py = k < 4 ? quad_from(yn, k) : m.yc[m2[k - 4]];
+        const double pz = k < 4 ? quad_from(zn, k) : m.zc[m2[k - 4]];
+        dx[k] = use ? (float)(px - xf) : 0.0f;
+        dy[k] = use ? (float)(py - yf) : 0.0f;
+        dz[k] = use ? (float)(pz - zf) : 0.0f;
         a11 += dx[k] * dx[k];
         a22 += dy[k] * dy[k];
         a33 += dz[k] * dz[k];
@@ -1733,8 +1830,9 @@ __device__ __forceinline__ void viscous_stencil_f32(const Params &P, const doubl
         for (int v = 0; v < 4; v++) {
             float b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
 #pragma unroll
-            for (int k = 0; k < NM; k++) {
-                const float dv = (float)(QV(slot[o + v], mm[k]) - fv[o + v]);
+            for (int k = 0; k < 8; k++) {
+                const double qk = k < 4 ? quad_from(qn[o + v], k) : QV(slot[o + v], m2[k - 4]);
+                const float dv = (float)(qk - fv[o + v]);
                 b1 += dv * dx[k];
                 b2 += dv * dy[k];
                 b3 += dv * dz[k];
@@ -1779,20 +1877,28 @@ __device__ __forceinline__ void viscous_face_f32(const Params &P, const State &s
         const double xf = m.fc3[(size_t)(3 * f) * ne + i], yf = m.fc3[(size_t)(3 * f + 1) * ne + i],
                      zf = m.fc3[(size_t)(3 * f + 2) * ne + i];
         const int neigh = m.nc[(size_t)f * ne + i];
+        double qn[8];
+        for (int v = 0; v < 8; v++) qn[v] = QV(slot[v], neigh);
+        const double xn = m.xc[neigh], yn = m.yc[neigh], zn = m.zc[neigh];
+        double vq[(Q_N + 3) / 4];
+        quad_load<Q_N>(q, nt, i, f, vq);
         double fv[8];
-        for (int v = 0; v < 8; v++) fv[v] = (QV(slot[v], i) + QV(slot[v], neigh)) * 0.5;
-        const float phigf = (float)((QV(Q_PHIG, i) + QV(Q_PHIG, neigh)) * 0.5);
-        const float phipf = (float)((QV(Q_PHIP, i) + QV(Q_PHIP, neigh)) * 0.5);
-        int mm[8];
-        for (int k = 0; k < 4; k++) mm[k] = m.nc[(size_t)k * ne + i];
-        if (neigh >= ne) {
-            mm[4] = i;
-            mm[5] = mm[6] = mm[7] = i;
-            viscous_stencil_f32<5>(P, q, nt, m, mm, xf, yf, zf, scx, scy, scz, fv, phigf, phipf, acc);
-        } else {
-            for (int k = 0; k < 4; k++) mm[4 + k] = m.nc[(size_t)k * ne + neigh];
-            viscous_stencil_f32<8>(P, q, nt, m, mm, xf, yf, zf, scx, scy, scz, fv, phigf, phipf, acc);
-        }
+        fv[0] = (quad_plane<Q_UG>(vq) + qn[0]) * 0.5;
+        fv[1] = (quad_plane<Q_VG>(vq) + qn[1]) * 0.5;
+        fv[2] = (quad_plane<Q_WG>(vq) + qn[2]) * 0.5;
+        fv[3] = (quad_plane<Q_TG>(vq) + qn[3]) * 0.5;
+        fv[4] = (quad_plane<Q_UP>(vq) + qn[4]) * 0.5;
+        fv[5] = (quad_plane<Q_VP>(vq) + qn[5]) * 0.5;
+        fv[6] = (quad_plane<Q_WP>(vq) + qn[6]) * 0.5;
+        fv[7] = (quad_plane<Q_TP>(vq) + qn[7]) * 0.5;
+        const float phigf = (float)((quad_plane<Q_PHIG>(vq) + QV(Q_PHIG, neigh)) * 0.5);
+        const float phipf = (float)((quad_plane<Q_PHIP>(vq) + QV(Q_PHIP, neigh)) * 0.5);
+        const bool ghost = neigh >= ne;
+        const int far = ghost ? i : neigh;
+        int m2[4];
+        for (int k = 0; k < 4; k++) m2[k] = ghost ? i : m.nc[(size_t)k * ne + far];
+        viscous_stencil_f32(P, q, nt, m, m2, ghost, xn, yn, zn, qn, xf, yf, zf, scx, scy, scz, fv, phigf,
+                            phipf, acc);
     }
     for (int v = 0; v < 8; v++) {
         const double t0 = (double)__shfl(acc[v], qbase, 64), t1 = (double)__shfl(acc[v], qbase + 1, 64),
