@@ -207,16 +207,28 @@ void enqueue_step(const Batch &B, const Grid &G, hipStream_t st, hipEvent_t *fev
     launch_smooth(B, G, st);       // :563
     launch_bc(B, G, 1, st);        // sr_RK.F90:19, viscousgp's BCs
     launch_viscous(B, G, st);
+    launch_eos(B, G, 0, st);       // stage 1's primitiveplusbc
+    // Each stage's flux launch ends with the primitive recovery the next
+    // stage starts with (the step's last one, main.F90:595, with cu = cn,
+    // :625; no monitor diffs here), written to the other primitive buffer:
+    // stages 1 and 3 read q, stages 2 and 4 read q2 (Bs), and the step ends
+    // in q, where it started.
+    Batch Bs = B;
+    Bs.swap = 1;
     for (int k = 1; k <= 4; k++) {
-        launch_eos(B, G, 0, st);   // primitiveplusbc
-        launch_bc(B, G, 0, st);
-        launch_grad(B, G, st);
+        const Batch &Bk = (k & 1) ? B : Bs;
+        launch_bc(Bk, G, 0, st);
+        launch_grad(Bk, G, st);
         if (fev) (void)hipEventRecord(fev[2 * (k - 1)], st);
-        launch_flux(B, G, k, st);  // face flux + source + stage k update
+        launch_flux(Bk, G, k, k == 4 ? 2 : 1, st);  // face flux + source + stage k update + primitives
         if (fev) (void)hipEventRecord(fev[2 * (k - 1) + 1], st);
-        if (D) launch_donor_gather(B.S1, *D, st);  // overset receptors of a lone case
+        if (D) {
+            // overset receptors of a lone case, in the buffer just written
+            State s = B.S1;
+            if (k & 1) std::swap(s.q, s.q2);
+            launch_donor_gather(B.P1, s, *D, k == 4 ? 2 : 1, st);
+        }
     }
-    launch_eos_end(B, G, st);      // main.F90:595, and cu = cn (:625); no monitor diffs here
 }
 
 }  // namespace
@@ -318,7 +330,7 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     ALLOC(m.sc, 12 * (size_t)ne) ALLOC(m.xc, ntot) ALLOC(m.yc, ntot) ALLOC(m.zc, ntot) ALLOC(m.vol, ntot)
     ALLOC(m.dl, ne) ALLOC(m.epssq, ne) ALLOC(m.dd, ne) ALLOC(m.fc0, 12 * (size_t)ne) ALLOC(m.fc3, 12 * (size_t)ne)
     ALLOC(s.cn, 10 * (size_t)ne) ALLOC(s.cu, 10 * (size_t)ne) ALLOC(s.cn1, 10 * (size_t)ne)
-    ALLOC(s.q, (size_t)Q_N * ntot) ALLOC(s.grad, (size_t)3 * G_N * ne) ALLOC(s.phi, ne)
+    ALLOC(s.q, (size_t)Q_N * ntot) ALLOC(s.q2, (size_t)Q_N * ntot) ALLOC(s.grad, (size_t)3 * G_N * ne) ALLOC(s.phi, ne)
     ALLOC(s.vis, 8 * (size_t)ne) ALLOC(s.d24, 10 * (size_t)ne) ALLOC(s.d2q, 10 * (size_t)ne)
     ALLOC(s.anu, 2 * (size_t)ne) ALLOC(s.rhs, 10 * (size_t)ne) ALLOC(s.scal, SC_N) ALLOC(s.itot, 1)
     ALLOC(c->health, 1)
@@ -529,7 +541,7 @@ static void flux_solver(hp_ctx *c, int stage)
     launch_eos(B, c->G, 0, c->st);  // primitiveplusbc, interior
     launch_bc(B, c->G, 0, c->st);   // primitiveplusbc, ghosts
     launch_grad(B, c->G, c->st);    // myorder 2 only
-    launch_flux(B, c->G, stage, c->st);
+    launch_flux(B, c->G, stage, 0, c->st);
 }
 
 int hp_flux(hp_ctx *c)
@@ -545,7 +557,7 @@ int hp_rk_stage(hp_ctx *c, int stage)
     if (!c || stage < 1 || stage > 4) return fail(c, HP_ERR_ARG, "hp_rk_stage: stage must be 1..4");
     if (int rc = sync_owner(c)) return rc;
     launch_rk_update(c->ts.view(), c->G, stage, c->st);
-    launch_donor_gather(c->s, c->D, c->st);
+    launch_donor_gather(c->P, c->s, c->D, 0, c->st);
     return finish(c, "rk_stage");
 }
 
@@ -558,7 +570,7 @@ int hp_rk(hp_ctx *c)
     for (int st = 1; st <= 4; st++) {
         flux_solver(c, 0);
         launch_rk_update(c->ts.view(), c->G, st, c->st);
-        launch_donor_gather(c->s, c->D, c->st);
+        launch_donor_gather(c->P, c->s, c->D, 0, c->st);
     }
     return finish(c, "rk");
 }
@@ -732,7 +744,26 @@ int hp_set_donors(hp_ctx *c, int n, const int *recv, const int *donors, int ld, 
     for (void *p : c->dbufs) HIPCHK(c, hipFree(p));
     c->dbufs.clear();
     c->D = Donors{};
-    if (n == 0) return HP_OK;
+    c->s.dflag = nullptr;
+    c->ts.hS[0].dflag = nullptr;
+    if (n == 0) return ts_upload(c->ts, c->st, c->err);
+    // the distinct donors and a per-cell donor flag (the fused step's flux
+    // leaves their cn to the gather)
+    std::vector<unsigned char> hflag(ne, 0);
+    for (int r = 0; r < n; r++) {
+        hflag[hid[r].x] = 1; hflag[hid[r].y] = 1; hflag[hid[r].z] = 1; hflag[hid[r].w] = 1;
+    }
+    std::vector<int> hdons;
+    for (int i = 0; i < ne; i++)
+        if (hflag[i]) hdons.push_back(i);
+    const int nd = (int)hdons.size();
+    void *pf = nullptr, *pd = nullptr;
+    HIPCHK(c, hipMalloc(&pf, ne));
+    c->dbufs.push_back(pf);
+    HIPCHK(c, hipMalloc(&pd, sizeof(int) * nd));
+    c->dbufs.push_back(pd);
+    HIPCHK(c, hipMemcpyAsync(pf, hflag.data(), ne, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(pd, hdons.data(), sizeof(int) * nd, hipMemcpyHostToDevice, c->st));
     void *pr = nullptr, *pi = nullptr, *pw = nullptr;
     HIPCHK(c, hipMalloc(&pr, sizeof(int) * n));
     c->dbufs.push_back(pr);
@@ -748,13 +779,19 @@ int hp_set_donors(hp_ctx *c, int n, const int *recv, const int *donors, int ld, 
     c->D.recv = static_cast<const int *>(pr);
     c->D.ids = static_cast<const int4 *>(pi);
     c->D.w = static_cast<const double2 *>(pw);
+    c->D.nd = nd;
+    c->D.dons = static_cast<const int *>(pd);
+    c->s.dflag = static_cast<const unsigned char *>(pf);
+    c->ts.hS[0].dflag = c->s.dflag;
+    if (int rc = ts_upload(c->ts, c->st, c->err)) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->st));
     return HP_OK;
 }
 
 int hp_apply_donors(hp_ctx *c)
 {
     if (!c) return fail(nullptr, HP_ERR_ARG, "null context");
-    launch_donor_gather(c->s, c->D, c->st);
+    launch_donor_gather(c->P, c->s, c->D, 0, c->st);
     return finish(c, "apply_donors");
 }
 
@@ -887,6 +924,7 @@ int hp_batch_add(hp_batch *b, hp_ctx *c, int *slot)
     }
     b->ts.hP[k] = c->P;
     b->ts.hS[k] = c->s;
+    b->ts.hS[k].dflag = nullptr;  // batch steps run no donor gather
     b->ts.hM[k] = c->m;
     b->ts.used[k] = 1;
     if (int rc = ts_upload(b->ts, b->st, b->err)) {
@@ -1028,7 +1066,7 @@ size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision)
     size_t b = 3 * (size_t)nodes * D;                        // coordinates
     b += (13 * ne + 4 * ng) * I;                             // nod, nc, ncp, iblank, ghost lists
     b += (12 + 3 + 12 + 12) * ne * D + 4 * nt * D;         // sc, dl/epssq/dd, fc0, fc3, xc..vol
-    b += (30 + 3 * G_N + 1 + 8 + 10 + 10 + 2 + 10) * ne * D + (size_t)Q_N * nt * D + SC_N * D + I;
+    b += (30 + 3 * G_N + 1 + 8 + 10 + 10 + 2 + 10) * ne * D + 2 * (size_t)Q_N * nt * D + SC_N * D + I;
     if (precision == 1) b += (3 * G_N + 1) * ne * sizeof(float);  // FP32 gradients, limiter
     else b += SRC_N * ne * D;                                     // interphase source terms
     const size_t tc = (ne + TILE - 1) / TILE, tg = (ng + TILE - 1) / TILE + 1;

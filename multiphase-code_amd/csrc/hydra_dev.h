@@ -43,6 +43,7 @@ struct State {
     int ne, nt;
     double *cn, *cu, *cn1;  // 10 x ne
     double *q;              // Q_N x nt
+    double *q2;             // Q_N x nt: the fused step's other primitive buffer
     double *grad;           // 3*G_N x ne
     double *phi;            // ne
     double *src;            // SRC_N x ne interphase source terms x volume (FP64 form; else null)
@@ -54,6 +55,7 @@ struct State {
     const double *dl;       // ne (same as Mesh::dl)
     double *scal;           // SC_N
     int *itot;              // total (step counter used by the outflow BC)
+    const unsigned char *dflag;  // ne: 1 at overset donors (null without donor lists)
 };
 
 // Work of one launch.  Cells (and boundary ghosts) of every case are cut
@@ -84,6 +86,9 @@ struct Batch {
     State S1;
     Mesh M1;
     int n1, n1g;
+    // fused step: this launch reads the q2 buffers as q (and writes q); the
+    // two primitive buffers trade places from one stage to the next
+    int swap;
 };
 // grid of each launch family (blocks; the tile loops take multiples of 8)
 // and the kernel variants shared by every case of the batch
@@ -105,10 +110,11 @@ void launch_bc(const Batch &B, const Grid &G, int viscous, hipStream_t st);
 void launch_smooth(const Batch &B, const Grid &G, hipStream_t st);
 void launch_viscous(const Batch &B, const Grid &G, hipStream_t st);
 void launch_grad(const Batch &B, const Grid &G, hipStream_t st);
-void launch_flux(const Batch &B, const Grid &G, int stage, hipStream_t st);
+// feos: 0 none; 1 fused step, the updated cells' primitive recovery into q2;
+// 2 as 1 plus cu = cn (last stage)
+void launch_flux(const Batch &B, const Grid &G, int stage, int feos, hipStream_t st);
 void launch_rk_update(const Batch &B, const Grid &G, int stage, hipStream_t st);
 void launch_end_step(const Batch &B, const Grid &G, int want_diffs, hipStream_t st);
-void launch_eos_end(const Batch &B, const Grid &G, hipStream_t st);  // primitive, then cu = cn
 void launch_health(const Batch &B, const Grid &G, int *flags, hipStream_t st);
 void launch_check_division(const double *a, const double *b, size_t n, unsigned long long *bad, hipStream_t st);
 void launch_dt_from_prims(const Batch &B, const Grid &G, hipStream_t st);
@@ -119,8 +125,12 @@ struct Donors {
     const int *recv;      // n
     const int4 *ids;      // n, 0-based
     const double2 *w;     // 2n: (w1, w2), (w3, w4)
+    int nd;               // distinct donors
+    const int *dons;      // nd, 0-based
 };
-void launch_donor_gather(const State &s, const Donors &D, hipStream_t st);
+// eos: 0 the gather alone; fused step, 1 then the receptors' and the donors'
+// primitive recovery into s.q, 2 as 1 plus cu = cn (last stage)
+void launch_donor_gather(const Params &P, const State &s, const Donors &D, int eos, hipStream_t st);
 
 }  // namespace hydra
 

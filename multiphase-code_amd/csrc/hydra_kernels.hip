@@ -99,6 +99,11 @@ __device__ __forceinline__ TileCase tile_case(const Batch &B, const Tile *tab, i
         c.first = T.first;
         c.count = T.count;
     }
+    if (B.swap) {
+        double *t = c.S.q;
+        c.S.q = c.S.q2;
+        c.S.q2 = t;
+    }
     return c;
 }
 
@@ -259,14 +264,12 @@ __global__ void k_geom_ghost(Mesh m, double xmax, double ymax, double zmax, doub
 // sr_primitiveplusbc.F90:36-172).  Optionally the CFL bound of
 // main.F90:541-545 reduced to one minimum per block with wave shuffles.
 // ---------------------------------------------------------------------------
-// cu (may be null): also store the rewritten conserved variables there, the
-// end-of-step cu = cn (main.F90:625) fused into the last primitive
-__device__ __forceinline__ void eos_cell(const Params &P, double *cn, int ne, int i, double *q, int nt,
-                                         double *dtloc, const double *dl, double *cu = nullptr)
+// Primitive recovery of one cell from its conserved variables c: the
+// primitives v (Q_ slots) and the conserved variables rewritten from them, nv
+__device__ __forceinline__ void eos_prims(const Params &P, const double (&c)[10], double (&v)[Q_N], double (&nv)[10])
 {
-    const double c1 = cn[i], c2 = cn[(size_t)ne + i], c3 = cn[(size_t)2 * ne + i], c4 = cn[(size_t)3 * ne + i],
-                 c5 = cn[(size_t)4 * ne + i], c6 = cn[(size_t)5 * ne + i], c7 = cn[(size_t)6 * ne + i],
-                 c8 = cn[(size_t)7 * ne + i], c9 = cn[(size_t)8 * ne + i], c10 = cn[(size_t)9 * ne + i];
+    const double c1 = c[0], c2 = c[1], c3 = c[2], c4 = c[3], c5 = c[4], c6 = c[5], c7 = c[6], c8 = c[7],
+                 c9 = c[8], c10 = c[9];
     double ug = c2 / c1, vg = c3 / c1, wg = c4 / c1;
     double up = c7 / c6, vp = c8 / c6, wp = c9 / c6;
     double eg = c5 / c1 - 0.5 * (ug * ug + vg * vg + wg * wg);
@@ -315,16 +318,30 @@ __device__ __forceinline__ void eos_cell(const Params &P, double *cn, int ne, in
     ep = (pp + P.gamap * P.pinf) / (pm1 * rop);
     const double egt = eg + 0.5 * (ug * ug + vg * vg + wg * wg);
     const double ept = ep + 0.5 * (up * up + vp * vp + wp * wp);
-    QV(Q_UG, i) = ug; QV(Q_VG, i) = vg; QV(Q_WG, i) = wg; QV(Q_PG, i) = pg; QV(Q_PP, i) = pp;
-    QV(Q_TG, i) = tg; QV(Q_PHIG, i) = phig; QV(Q_UP, i) = up; QV(Q_VP, i) = vp; QV(Q_WP, i) = wp;
-    QV(Q_PHIP, i) = phip; QV(Q_TP, i) = tp; QV(Q_ROP, i) = rop; QV(Q_ROG, i) = rog;
+    v[Q_UG] = ug; v[Q_VG] = vg; v[Q_WG] = wg; v[Q_PG] = pg; v[Q_PP] = pp;
+    v[Q_TG] = tg; v[Q_PHIG] = phig; v[Q_UP] = up; v[Q_VP] = vp; v[Q_WP] = wp;
+    v[Q_PHIP] = phip; v[Q_TP] = tp; v[Q_ROP] = rop; v[Q_ROG] = rog;
     const double mg = phig * rog, mp = phip * rop;
-    const double nv[10] = {mg, mg * ug, mg * vg, mg * wg, mg * egt, mp, mp * up, mp * vp, mp * wp, mp * ept};
+    nv[0] = mg; nv[1] = mg * ug; nv[2] = mg * vg; nv[3] = mg * wg; nv[4] = mg * egt;
+    nv[5] = mp; nv[6] = mp * up; nv[7] = mp * vp; nv[8] = mp * wp; nv[9] = mp * ept;
+}
+
+// cu (may be null): also store the rewritten conserved variables there, the
+// end-of-step cu = cn (main.F90:625) fused into the last primitive recovery
+__device__ __forceinline__ void eos_cell(const Params &P, double *cn, int ne, int i, double *q, int nt,
+                                         double *dtloc, const double *dl, double *cu = nullptr)
+{
+    double c[10], v[Q_N], nv[10];
+    for (int k = 0; k < 10; k++) c[k] = cn[(size_t)k * ne + i];
+    eos_prims(P, c, v, nv);
+    for (int k = 0; k < Q_N; k++) QV(k, i) = v[k];
     for (int k = 0; k < 10; k++) cn[(size_t)k * ne + i] = nv[k];
     if (cu)
         for (int k = 0; k < 10; k++) cu[(size_t)k * ne + i] = nv[k];
     if (dtloc) {
         // main.F90:541-545
+        const double pg = v[Q_PG], pp = v[Q_PP], rog = v[Q_ROG], rop = v[Q_ROP];
+        const double ug = v[Q_UG], vg = v[Q_VG], wg = v[Q_WG], up = v[Q_UP], vp = v[Q_VP], wp = v[Q_WP];
         double ag = sqrt((P.gamag * pg) / rog);
         double ap = sqrt((P.gamap * (pp + P.pinf)) / rop);
         double qgi = sqrt(ug * ug + vg * vg + wg * wg);
@@ -1093,12 +1110,13 @@ __device__ __forceinline__ void hllc_zien(const Params &P, const FState &L, cons
 
 // the residual r of element o (variable-major offset): stored in rhs
 // (stage 0, per-routine checks) or applied as RK stage `stage`
-// (sr_RK.F90:29-161); FP64 in every precision
-__device__ __forceinline__ void rk_finish(const State &s, size_t o, double r, int stage, double dtbyvol)
+// (sr_RK.F90:29-161), whose new value is returned and, with `store`,
+// written to cn; FP64 in every precision
+__device__ __forceinline__ double rk_finish(const State &s, size_t o, double r, int stage, double dtbyvol, bool store)
 {
     if (stage == 0) {
         s.rhs[o] = r;
-        return;
+        return 0.0;
     }
     const double rr = r - s.d24[o];
     const double cu = s.cu[o];
@@ -1114,22 +1132,26 @@ __device__ __forceinline__ void rk_finish(const State &s, size_t o, double r, in
         cn = (111.0 / 1331.0) * cu + (260.0 / 1331.0) * s.cn1[o] +
              (960.0 / 1331.0) * (cn - (11.0 / 20.0) * dtbyvol * rr);
     }
-    s.cn[o] = cn;
+    if (store) s.cn[o] = cn;
+    return cn;
 }
 
 // RK stage update (sr_RK.F90:29-161) of the quad's cell i from its
 // face-summed residual and the stored interphase source terms (source_cell;
 // FP64 form).  Lane f of the quad finishes the conserved variables k = f,
-// f+4, f+8, whose face sums are rhs[0], rhs[1], rhs[2].  stage >= 1: update
-// cn (and cn1); stage == 0: store the residual in rhs (per-routine checks).
+// f+4, f+8, whose face sums are rhs[0], rhs[1], rhs[2].  stage >= 1: the
+// new values go to cv[0..2] and, with `store`, to cn (cn1 at stage 1);
+// stage == 0: store the residual in rhs (per-routine checks).
 __device__ __forceinline__ void source_and_update(const State &s, const Mesh &m, int i, int f,
-                                                  const double *rhs, int stage)
+                                                  const double *rhs, int stage, bool store, double (&cv)[3])
 {
     const int ne = m.ne;
     const double *vis = s.vis, *src = s.src;
     const double dtbyvol = (stage > 0) ? s.scal[SC_DT] / m.vol[i] : 0.0;
-    for (int k = f; k < 10; k += 4) {
-        double r = rhs[k >> 2];
+    for (int j = 0; j < 3; j++) {
+        const int k = f + 4 * j;
+        if (k >= 10) break;
+        double r = rhs[j];
         switch (k) {
         case 1: r = r - src[(size_t)SRC_MX * ne + i] - vis[(size_t)0 * ne + i]; break;
         case 2: r = r - src[(size_t)SRC_MY * ne + i] - vis[(size_t)1 * ne + i]; break;
@@ -1141,8 +1163,34 @@ __device__ __forceinline__ void source_and_update(const State &s, const Mesh &m,
         case 9: r = r + src[(size_t)SRC_E * ne + i] - vis[(size_t)7 * ne + i]; break;
         default: break;
         }
-        rk_finish(s, (size_t)k * ne + i, r, stage, dtbyvol);
+        cv[j] = rk_finish(s, (size_t)k * ne + i, r, stage, dtbyvol, store);
     }
+}
+
+// Fused step: the primitive recovery of cell i right after its stage
+// update, i.e. the EOS pass the next stage (or the end of the step) would
+// run on it.  The quads of a tile leave their cells' new conserved
+// variables in LDS (c: variable k of the thread's cell at c[k * TILE]); one
+// thread per cell recovers the primitives into the other primitive buffer
+// q2 (the next stage's q) and stores the rewritten conserved variables in
+// cn, and in cu as well at the end of the step (`end`).  An overset donor
+// stores its stage update unrewritten: the donor gather reads it, and the
+// donors' own rewrite follows the gather (k_donor_eos).
+__device__ __forceinline__ void eos_staged(const Params &P, const State &s, int i, const double *c_lds, bool end)
+{
+    double c[10], v[Q_N], nv[10];
+    for (int k = 0; k < 10; k++) c[k] = c_lds[k * TILE];
+    eos_prims(P, c, v, nv);
+    const int ne = s.ne;
+    const size_t nt = s.nt;
+    for (int k = 0; k < Q_N; k++) s.q2[(size_t)k * nt + i] = v[k];
+    if (s.dflag && s.dflag[i]) {
+        for (int k = 0; k < 10; k++) s.cn[(size_t)k * ne + i] = c[k];
+        return;
+    }
+    for (int k = 0; k < 10; k++) s.cn[(size_t)k * ne + i] = nv[k];
+    if (end)
+        for (int k = 0; k < 10; k++) s.cu[(size_t)k * ne + i] = nv[k];
 }
 
 // One cell per quad of lanes, one face per lane (sr_hllcbatten.F90:77-555):
@@ -1153,12 +1201,10 @@ __device__ __forceinline__ void source_and_update(const State &s, const Mesh &m,
 // Cell i's primitives and gradients are loaded once per quad (quad_load) and
 // handed to the four lanes by DPP.
 template <int FLUX>
-__device__ __forceinline__ void flux_face(const Params &P, const State &s, const Mesh &m, int i, int f, int qbase,
-                                          int stage)
+__device__ __forceinline__ void flux_update(const Params &P, const State &s, const Mesh &m, int i, int f,
+                                            int qbase, int stage, bool store, double (&cv)[3])
 {
     const int ne = m.ne;
-    // all four lanes of a quad share i, so a quad leaves together
-    if (m.iblank[i] != 1) return;
     const double *q = s.q;
     const size_t nt = s.nt;
     const double *s_grad = s.grad;
@@ -1243,7 +1289,26 @@ __device__ __forceinline__ void flux_face(const Params &P, const State &s, const
         const double sum = 0.0 + t0 + t1 + t2 + t3;
         if ((k & 3) == f) rhs[k >> 2] = sum;
     }
-    source_and_update(s, m, i, f, rhs, stage);
+    source_and_update(s, m, i, f, rhs, stage, store, cv);
+}
+
+// Face fluxes and stage update of cell i (flux_update) for an active cell.
+// feos (fused step): the update is not stored; lane f returns the cell's
+// new conserved variables f, f+4, f+8 in cv (a frozen cell its unchanged
+// ones) for the primitive recovery (eos_staged).  All four lanes of a quad
+// share i, so the quad takes one branch.
+template <int FLUX>
+__device__ __forceinline__ void flux_face(const Params &P, const State &s, const Mesh &m, int i, int f, int qbase,
+                                          int stage, int feos, double (&cv)[3])
+{
+    if (m.iblank[i] == 1) {
+        flux_update<FLUX>(P, s, m, i, f, qbase, stage, feos == 0, cv);
+    } else if (feos) {
+        const int ne = m.ne;
+        cv[0] = s.cn[(size_t)f * ne + i];
+        cv[1] = s.cn[(size_t)(f + 4) * ne + i];
+        if (f + 8 < 10) cv[2] = s.cn[(size_t)(f + 8) * ne + i];
+    }
 }
 
 // stage update from a stored residual (sr_RK.F90), for the per-routine path
@@ -1644,7 +1709,8 @@ __device__ __forceinline__ float quad_sum4f(float v, int qbase)
 // widened to FP64 where they join the FP64 residual, and the RK update is
 // the FP64 one
 __device__ __forceinline__ void source_and_update_f32(const Params &P, const State &s, const Mesh &m, int i,
-                                                      int f, int qbase, int neigh, const double *rhs, int stage)
+                                                      int f, int qbase, int neigh, const double *rhs, int stage,
+                                                      bool store, double (&cv)[3])
 {
 #pragma clang fp contract(fast)
     const int ne = m.ne;
@@ -1696,8 +1762,10 @@ __device__ __forceinline__ void source_and_update_f32(const Params &P, const Sta
                        wint * fdgz + qsource;
     const float grav = (float)P.gravity;
     const double dtbyvol = (stage > 0) ? s.scal[SC_DT] / voli : 0.0;
-    for (int k = f; k < 10; k += 4) {
-        double r = rhs[k >> 2];
+    for (int j = 0; j < 3; j++) {
+        const int k = f + 4 * j;
+        if (k >= 10) break;
+        double r = rhs[j];
         const double *vis = s.vis;
         switch (k) {
         case 1: r = r - (double)(fdgx + pint * dphx + phig * rog * grav) * voli - vis[(size_t)0 * ne + i]; break;
@@ -1710,7 +1778,7 @@ __device__ __forceinline__ void source_and_update_f32(const Params &P, const Sta
         case 9: r = r + (double)work * voli - vis[(size_t)7 * ne + i]; break;
         default: break;
         }
-        rk_finish(s, (size_t)k * ne + i, r, stage, dtbyvol);
+        cv[j] = rk_finish(s, (size_t)k * ne + i, r, stage, dtbyvol, store);
     }
 }
 
@@ -1720,12 +1788,11 @@ __device__ __forceinline__ void source_and_update_f32(const Params &P, const Sta
 // face fluxes are widened to FP64 and summed in face order, then the FP64
 // source and stage update run unchanged.
 template <int FLUX>
-__device__ __forceinline__ void flux_face_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
-                                              int qbase, int stage)
+__device__ __forceinline__ void flux_update_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
+                                                int qbase, int stage, bool store, double (&cv)[3])
 {
 #pragma clang fp contract(fast)
     const int ne = m.ne;
-    if (m.iblank[i] != 1) return;
     const double *q = s.q;
     const size_t nt = s.nt;
     const int neigh = m.nc[(size_t)f * ne + i];
@@ -1781,7 +1848,22 @@ __device__ __forceinline__ void flux_face_f32(const Params &P, const State &s, c
         const double sum = 0.0 + t0 + t1 + t2 + t3;
         if ((k & 3) == f) rhs[k >> 2] = sum;
     }
-    source_and_update_f32(P, s, m, i, f, qbase, neigh, rhs, stage);
+    source_and_update_f32(P, s, m, i, f, qbase, neigh, rhs, stage, store, cv);
+}
+
+// flux_face with the FP32-bulk face loop
+template <int FLUX>
+__device__ __forceinline__ void flux_face_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
+                                              int qbase, int stage, int feos, double (&cv)[3])
+{
+    if (m.iblank[i] == 1) {
+        flux_update_f32<FLUX>(P, s, m, i, f, qbase, stage, feos == 0, cv);
+    } else if (feos) {
+        const int ne = m.ne;
+        cv[0] = s.cn[(size_t)f * ne + i];
+        cv[1] = s.cn[(size_t)(f + 4) * ne + i];
+        if (f + 8 < 10) cv[2] = s.cn[(size_t)(f + 8) * ne + i];
+    }
 }
 
 // viscous_stencil in FP32, with the quad's shared points as there:
@@ -1924,7 +2006,7 @@ __device__ __forceinline__ double wave_max(double v)
 
 // primitive (sr_primitive.F90:18-178) on interior cells; want_dt: also the
 // CFL bound per tile (main.F90:541-545)
-// KIND 0: primitives; 1: and the CFL bound per tile; 2: and cu = cn
+// KIND 0: primitives; 1: and the CFL bound per tile
 template <int MODE, int KIND>
 __global__ void __launch_bounds__(TILE) k_eos(Batch B)
 {
@@ -1935,8 +2017,7 @@ __global__ void __launch_bounds__(TILE) k_eos(Batch B)
         const State &s = C.S;
         double loc = 1.0e8;
         if ((int)threadIdx.x < C.count)
-            eos_cell(C.P, s.cn, s.ne, C.first + (int)threadIdx.x, s.q, s.nt, want_dt ? &loc : nullptr, s.dl,
-                     KIND == 2 ? s.cu : nullptr);
+            eos_cell(C.P, s.cn, s.ne, C.first + (int)threadIdx.x, s.q, s.nt, want_dt ? &loc : nullptr, s.dl);
         if (want_dt) {
             const double v = wave_min(loc);
             if (threadIdx.x == 0) B.part[(size_t)4 * t] = v;
@@ -1990,8 +2071,16 @@ __global__ void __launch_bounds__(TILE) k_bc(Batch B, int viscous)
     TILE_LOOP(B.cnt[1], B.n1g)
     {
         const TileCase C = tile_case<MODE>(B, B.gt, t, B.n1g);
-        if ((int)threadIdx.x < C.count)
-            bc_ghost(C.P, C.S, C.M, C.first + (int)threadIdx.x, viscous);
+        if ((int)threadIdx.x < C.count) {
+            const int g = C.first + (int)threadIdx.x;
+            bc_ghost(C.P, C.S, C.M, g, viscous);
+            // The fused step alternates two primitive buffers; a ghost's state
+            // goes to both, so the relaxing BCs (types 3, 51, 53) always read
+            // the ghost's latest value, and the buffer a step ends in holds
+            // the last stage's ghosts.
+            const size_t nt = C.S.nt, o = (size_t)C.S.ne + g;
+            for (int k = 0; k < Q_N; k++) C.S.q2[(size_t)k * nt + o] = C.S.q[(size_t)k * nt + o];
+        }
     }
 }
 
@@ -2062,24 +2151,41 @@ __global__ void __launch_bounds__(4 * TILE) k_viscous(Batch B)
 // (2.07 ms, against 2.49 ms unconstrained and 2.84 ms with a reservation
 // allowing four; the same 48 KB requested as dynamic LDS at launch, which
 // the compiler does not see, measured 3.46 ms).
+// The fused step's staging of a tile's new conserved variables (eos_staged)
+// uses the first 10 x TILE doubles of the same reservation.
 constexpr int FLUX_F32_LDS_CAP = 48 * 1024;
+constexpr int FLUX_STAGE_DOUBLES = 10 * TILE;
+static_assert(FLUX_STAGE_DOUBLES * 8 <= FLUX_F32_LDS_CAP, "staging fits the FP32 reservation");
 
+// feos (fused step): 0 none; 1 each cell's primitive recovery right after
+// its stage update, by the block's first wave from the tile's updated cells
+// staged in LDS (eos_staged); 2 as 1 plus cu = cn (last stage)
 template <int MODE, int FLUX, bool F32>
-__global__ void __launch_bounds__(4 * TILE) k_flux(Batch B, int stage)
+__global__ void __launch_bounds__(4 * TILE) k_flux(Batch B, int stage, int feos)
 {
-    if constexpr (F32) {
-        __shared__ char cap[FLUX_F32_LDS_CAP];
-        if (stage < 0) cap[threadIdx.x] = 0;  // never taken (stage >= 0): keeps the reservation
-    }
+    __shared__ double stage_lds[F32 ? FLUX_F32_LDS_CAP / 8 : FLUX_STAGE_DOUBLES];
+    if (F32 && stage < 0) stage_lds[threadIdx.x] = 0.0;  // never taken (stage >= 0): keeps the FP32 reservation
     TILE_LOOP(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
         const int l = (int)(threadIdx.x >> 2), f = (int)(threadIdx.x & 3), qbase = (int)(threadIdx.x & 63) & ~3;
         if (l < C.count) {
+            double cv[3] = {0.0, 0.0, 0.0};
             if (F32)
-                flux_face_f32<FLUX>(C.P, C.S, C.M, C.first + l, f, qbase, stage);
+                flux_face_f32<FLUX>(C.P, C.S, C.M, C.first + l, f, qbase, stage, feos, cv);
             else
-                flux_face<FLUX>(C.P, C.S, C.M, C.first + l, f, qbase, stage);
+                flux_face<FLUX>(C.P, C.S, C.M, C.first + l, f, qbase, stage, feos, cv);
+            if (feos) {
+                stage_lds[f * TILE + l] = cv[0];
+                stage_lds[(f + 4) * TILE + l] = cv[1];
+                if (f + 8 < 10) stage_lds[(f + 8) * TILE + l] = cv[2];
+            }
+        }
+        if (feos) {
+            __syncthreads();
+            if ((int)threadIdx.x < C.count)
+                eos_staged(C.P, C.S, C.first + (int)threadIdx.x, stage_lds + threadIdx.x, feos == 2);
+            if (MODE == 1 && t + st_ < e_) __syncthreads();  // the next tile reuses the staging
         }
     }
 }
@@ -2154,10 +2260,6 @@ void launch_eos(const Batch &B, const Grid &G, int want_dt, hipStream_t st)
         hipLaunchKernelGGL(PICK(1, k_eos, 0), dim3(G.cell), dim3(TILE), 0, st, B);
 }
 
-void launch_eos_end(const Batch &B, const Grid &G, hipStream_t st)
-{
-    hipLaunchKernelGGL(PICK(1, k_eos, 2), dim3(G.cell), dim3(TILE), 0, st, B);
-}
 
 void launch_dt_final(const Batch &B, const Grid &G, hipStream_t st)
 {
@@ -2200,15 +2302,15 @@ void launch_grad(const Batch &B, const Grid &G, hipStream_t st)
     }
 }
 
-void launch_flux(const Batch &B, const Grid &G, int stage, hipStream_t st)
+void launch_flux(const Batch &B, const Grid &G, int stage, int feos, hipStream_t st)
 {
     const dim3 g(G.quad), b(4 * TILE);
     if (G.precision == 1) {
-        if (G.myflux == 1) hipLaunchKernelGGL(PICK(2, k_flux, 1, true), g, b, 0, st, B, stage);
-        else hipLaunchKernelGGL(PICK(2, k_flux, 2, true), g, b, 0, st, B, stage);
+        if (G.myflux == 1) hipLaunchKernelGGL(PICK(2, k_flux, 1, true), g, b, 0, st, B, stage, feos);
+        else hipLaunchKernelGGL(PICK(2, k_flux, 2, true), g, b, 0, st, B, stage, feos);
     } else {
-        if (G.myflux == 1) hipLaunchKernelGGL(PICK(2, k_flux, 1, false), g, b, 0, st, B, stage);
-        else hipLaunchKernelGGL(PICK(2, k_flux, 2, false), g, b, 0, st, B, stage);
+        if (G.myflux == 1) hipLaunchKernelGGL(PICK(2, k_flux, 1, false), g, b, 0, st, B, stage, feos);
+        else hipLaunchKernelGGL(PICK(2, k_flux, 2, false), g, b, 0, st, B, stage, feos);
     }
 }
 
@@ -2247,7 +2349,7 @@ void launch_health(const Batch &B, const Grid &G, int *flags, hipStream_t st)
 // overset: each receptor's conserved variables from its four donors,
 // w1*q1 + w2*q2 + w3*q3 + w4*q4 in that order (receptors are never donors,
 // so the reads never see this launch's writes)
-__global__ void __launch_bounds__(256) k_donor_gather(State s, Donors D)
+__global__ void __launch_bounds__(256) k_donor_gather(Params P, State s, Donors D, int eos)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= D.n) return;
@@ -2258,11 +2360,25 @@ __global__ void __launch_bounds__(256) k_donor_gather(State s, Donors D)
         const double *c = s.cn + (size_t)k * ne;
         s.cn[(size_t)k * ne + i] = wa.x * c[d.x] + wa.y * c[d.y] + wb.x * c[d.z] + wb.y * c[d.w];
     }
+    // fused step: the receptor's primitive recovery into s.q (the next
+    // stage's buffer), with cu = cn at the end of the step (eos == 2)
+    if (eos) eos_cell(P, s.cn, ne, i, s.q, s.nt, nullptr, s.dl, eos == 2 ? s.cu : nullptr);
 }
 
-void launch_donor_gather(const State &s, const Donors &D, hipStream_t st)
+// fused step, after the gather: the donors' primitive recovery, which the
+// flux epilogue left out of cn so that the gather read the stage update
+__global__ void __launch_bounds__(256) k_donor_eos(Params P, State s, Donors D, int end)
 {
-    if (D.n > 0) hipLaunchKernelGGL(k_donor_gather, dim3((D.n + 255) / 256), dim3(256), 0, st, s, D);
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= D.nd) return;
+    eos_cell(P, s.cn, s.ne, D.dons[r], s.q, s.nt, nullptr, s.dl, end ? s.cu : nullptr);
+}
+
+void launch_donor_gather(const Params &P, const State &s, const Donors &D, int eos, hipStream_t st)
+{
+    if (D.n > 0) hipLaunchKernelGGL(k_donor_gather, dim3((D.n + 255) / 256), dim3(256), 0, st, P, s, D, eos);
+    if (eos && D.nd > 0)
+        hipLaunchKernelGGL(k_donor_eos, dim3((D.nd + 255) / 256), dim3(256), 0, st, P, s, D, eos == 2 ? 1 : 0);
 }
 
 }  // namespace hydra
