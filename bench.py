@@ -48,8 +48,10 @@ HBM_PEAK_GBS = 8000.0
 # centroids 12x8, centroid 3x8, neighbour ids 4x4, vol 8, interphase source
 # terms 5x8, viscous flux 8x8, d24 10x8, cu 10x8, cn read (stages 2-4)
 # 3/4x10x8, cn1 read (stage 4) and write (stage 1) 2x1/4x10x8, cn write
-# 10x8.  Neighbour values are counted once, by their owning cell.
-FLUX_BYTES_PER_CELL = 112 + 264 + 8 + 96 + 96 + 24 + 16 + 8 + 40 + 64 + 80 + 80 + 60 + 40 + 80
+# 10x8; and the fused primitive recovery of the updated cells: the next
+# stage's primitives 14x8 and, at the last stage, cu 1/4x10x8.  Neighbour
+# values are counted once, by their owning cell.
+FLUX_BYTES_PER_CELL = 112 + 264 + 8 + 96 + 96 + 24 + 16 + 8 + 40 + 64 + 80 + 80 + 60 + 40 + 80 + 112 + 20
 # FP32-bulk form: gradients 33x4 and limiter 4 instead of 33x8 and 8; the
 # source is formed in the flux from vol+dl (16) instead of read (8 + 40)
 FLUX_BYTES_PER_CELL_F32 = FLUX_BYTES_PER_CELL - 132 - 4 + 16 - 48
