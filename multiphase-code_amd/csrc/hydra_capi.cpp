@@ -431,10 +431,13 @@ int hp_upload_state(hp_ctx *c, const double *cu, int ld, const double *gp, doubl
                                  hipMemcpyHostToDevice, c->st));
     }
     HIPCHK(c, hipMemcpyAsync(c->s.cn, c->s.cu, sizeof(double) * 10 * ne, hipMemcpyDeviceToDevice, c->st));
+    // the ghost state goes to both primitive buffers (the fused step's BCs
+    // relax from the ghost's previous value in whichever buffer they read)
     const int slot[6] = {Q_UG, Q_VG, Q_WG, Q_PG, Q_TG, Q_ROG};
     for (int k = 0; k < 6 && ng > 0; k++)
-        HIPCHK(c, hipMemcpyAsync(c->s.q + (size_t)slot[k] * c->s.nt + ne, gp + (size_t)k * ng, sizeof(double) * ng,
-                                 hipMemcpyHostToDevice, c->st));
+        for (double *qb : {c->s.q, c->s.q2})
+            HIPCHK(c, hipMemcpyAsync(qb + (size_t)slot[k] * c->s.nt + ne, gp + (size_t)k * ng, sizeof(double) * ng,
+                                     hipMemcpyHostToDevice, c->st));
     // main.F90:456-476: viscous fluxes and dissipation start at zero
     HIPCHK(c, hipMemsetAsync(c->s.vis, 0, sizeof(double) * 8 * ne, c->st));
     HIPCHK(c, hipMemsetAsync(c->s.d24, 0, sizeof(double) * 10 * ne, c->st));
