@@ -94,8 +94,11 @@ def test_batch_fp64_case_matches_oracle():
         assert np.array_equal(cu_o, g[0])
 
 
-def test_launch_shapes_give_same_results():
-    cases = [HB.build_case(s, 1) for s in specs()]
+@pytest.mark.parametrize("precision", [0, 1])
+def test_launch_shapes_give_same_results(precision):
+    """Persistent grids (blocks looping over several tiles, with the flux's
+    staged primitive recovery between tiles) equal one block per tile."""
+    cases = [HB.build_case(s, precision) for s in specs()]
     ref = in_batch(cases, 3)
     for grid in [(8, 8, 8), (16, 24, 8), (512, 1024, 64)]:
         got = in_batch(cases, 3, grid=grid)
