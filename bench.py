@@ -292,11 +292,12 @@ def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
 
 
 def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
-    """Poisson arrivals of cases (n 10..70, 50..500 steps), admitted by a
-    200 GB HBM budget and stepped together (FP32 bulk)."""
+    """Poisson arrivals of cases (n 10..70, 50..500 steps), admitted smallest
+    work first within a 200 GB HBM budget while at most 1M active cells are
+    resident, and stepped together (FP32 bulk)."""
     from hydraphase import serve as SV
     reqs = SV.poisson_requests(n_requests, rate=rate, seed=7, nmin=10, nmax=70, steps=(50, 500), precision=1)
-    srv = SV.Server(budget_bytes=200e9, max_cases=256, chunk=25)
+    srv = SV.Server(budget_bytes=200e9, max_cases=256, chunk=25, policy="shortest", fill_cells=1_000_000)
     try:
         st = srv.run(reqs)
     finally:
@@ -309,7 +310,8 @@ def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
         errs.append(float(SV.mixture_rel_l2(r.result[0], ref[0]).max()))
     cells = sum(int(r.mesh.iblank.sum()) for r in reqs)
     return {"workload": f"{n_requests} requests, Poisson {rate}/s, boxes n 10..70 ({cells} tets in all), "
-                        "50..500 steps, 200 GB budget, FP32 bulk",
+                        "50..500 steps, 200 GB budget, smallest work first within 1M resident active cells, "
+                        "FP32 bulk",
             "updates_per_s": st["updates_per_s"], "tts_p50_s": st["tts_p50_s"], "tts_p99_s": st["tts_p99_s"],
             "makespan_s": st["makespan_s"], "gpu_busy_s": st["gpu_busy_s"],
             "peak_resident_gb": st["peak_resident_bytes"] / 1e9, "requests_failed": st["failed"],

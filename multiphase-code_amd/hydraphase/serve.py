@@ -3,8 +3,12 @@
 Requests arrive (Poisson) carrying a case: mesh, parameters, initial state
 and the number of steps to run.  The loop
 
-* admits waiting requests in arrival order while a case slot is free and
-  their device bytes (hp_case_bytes) fit in the HBM budget,
+* admits waiting requests while a case slot is free and their device
+  bytes (hp_case_bytes) fit in the HBM budget: in arrival order (policy
+  "fifo") or smallest work first (policy "shortest": active cells x steps,
+  ties by arrival), and, with a fill target, only while the resident
+  active cells stay within it (a request above the target on its own runs
+  when nothing else is resident),
 * keeps every admitted case resident on the device,
 * advances all resident cases together, one hp_batch_step of at most
   `chunk` steps at a time (each case with its own dt), and
@@ -17,7 +21,12 @@ and the number of steps to run.  The loop
 
 Requests that arrive while others are running join at the next chunk
 boundary, so their work overlaps on the device with the cases already
-there.  A case's arithmetic reads only its own data, so its result is the
+there.  Every resident case advances by a chunk whatever its size, so a
+small request shares each step with the whole batch; with a fill target
+that just saturates the GPU, smallest-first admission lets short requests
+through without holding the GPU below full use.  Shortest-first can hold a
+large request back while smaller ones keep arriving; the runs here end
+their arrivals, so every request is served.  A case's arithmetic reads only its own data, so its result is the
 same bit for bit whether it ran alone or in any mix.
 
 Reported: throughput in cell-stage updates/s over the makespan, the
@@ -75,7 +84,11 @@ def poisson_requests(n: int, rate: float = 20.0, seed: int = 1, nmin: int = 10, 
 class Server:
     def __init__(self, budget_bytes: float = 200e9, max_cases: int = 256, chunk: int = 25,
                  graph: bool = False, grid=(0, 0, 0), bytes_per_cell: float = 1000.0,
-                 check_health: bool = True):
+                 check_health: bool = True, policy: str = "fifo", fill_cells: int | None = None):
+        if policy not in ("fifo", "shortest"):
+            raise ValueError(f"unknown admission policy {policy!r} (fifo or shortest)")
+        self.policy = policy
+        self.fill_cells = None if fill_cells is None else int(fill_cells)
         self.budget = int(budget_bytes)
         self.chunk = int(chunk)
         self.graph = graph
@@ -95,7 +108,9 @@ class Server:
         slot_of: dict[int, int] = {}
         failed = 0
         used = 0
+        resident = 0  # active cells of the admitted cases
         peak = 0
+        cells_of = lambda r: int(r.mesh.iblank.sum())
         updates = 0
         gpu_ms = 0.0
         for r in requests:
@@ -107,13 +122,18 @@ class Server:
             now = clock()
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
+            if self.policy == "shortest":
+                waiting = collections.deque(sorted(waiting, key=lambda r: (cells_of(r) * r.spec.steps, r.arrival)))
             while waiting and len(active) < self.max_cases and used + waiting[0].nbytes <= self.budget:
+                if self.fill_cells is not None and active and resident + cells_of(waiting[0]) > self.fill_cells:
+                    break
                 r = waiting.popleft()
                 s = Solver(r.mesh, r.params)
                 s.upload(r.cu, r.gp, 0.0, 0)
                 slot_of[r.rid] = self.batch.add(s)
                 active[r.rid] = (r, s, r.spec.steps)
                 used += r.nbytes
+                resident += cells_of(r)
                 peak = max(peak, used)
                 r.admit = clock()
             if not active:
@@ -145,6 +165,7 @@ class Server:
                 del active[rid]
                 del slot_of[rid]
                 used -= r.nbytes
+                resident -= cells_of(r)
                 r.finish = clock()
                 if log:
                     log(f"request {r.rid} (n={r.spec.n}, {r.spec.steps} steps) "

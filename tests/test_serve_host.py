@@ -1,11 +1,13 @@
 """The request loop's host logic (hydraphase.serve.Server.run) on the CPU,
 with stand-in batch and solver objects: admission in arrival order within
-the HBM budget and the slot limit, chunked stepping, retirement when done,
-retirement of a case whose health check fails, and the statistics."""
+the HBM budget and the slot limit, smallest-work-first admission and the
+fill target, chunked stepping, retirement when done, retirement of a case
+whose health check fails, and the statistics."""
 import math
 import types
 
 import numpy as np
+import pytest
 
 from hydraphase import batch as HB
 from hydraphase import serve as S
@@ -70,11 +72,11 @@ class FakeBatch:
         pass
 
 
-def requests(nbytes, steps, bad=()):
+def requests(nbytes, steps, bad=(), cells=None):
     out = []
     for k, (b, st) in enumerate(zip(nbytes, steps)):
         spec = HB.CaseSpec(4, 0.3, 0.1, False, k, steps=st)
-        mesh = types.SimpleNamespace(iblank=np.ones(10, dtype=np.int32))
+        mesh = types.SimpleNamespace(iblank=np.ones(10 if cells is None else cells[k], dtype=np.int32))
         params = {"bad_after": 3 if k in bad else None}
         out.append(S.Request(k, spec, 0.0, mesh, params, np.full((10, 10), float(k)), None, b))
     return out
@@ -120,3 +122,41 @@ def test_health_check_can_be_disabled(monkeypatch):
     reqs = requests([10, 10], [6, 6], bad=(0,))
     stats, _ = run(monkeypatch, reqs, budget_bytes=1000, max_cases=4, chunk=3, check_health=False)
     assert stats["failed"] == 0 and all(r.result[3] == 6 for r in reqs)
+
+
+def test_shortest_work_first(monkeypatch):
+    """All requests wait at the start; one slot: they are served smallest
+    active cells x steps first (ties by arrival), and each still runs
+    exactly its own steps."""
+    cells = [10, 10, 30, 5, 10]
+    steps = [8, 2, 4, 4, 2]      # work 80, 20, 120, 20, 20
+    reqs = requests([10] * 5, steps, cells=cells)
+    stats, b = run(monkeypatch, reqs, budget_bytes=1000, max_cases=1, chunk=2, policy="shortest")
+    order = [r.rid for r in sorted(reqs, key=lambda r: r.admit)]
+    assert order == [1, 3, 4, 0, 2]
+    assert all(r.error is None and r.result[3] == r.spec.steps for r in reqs)
+    assert stats["updates"] == sum(4 * c * st for c, st in zip(cells, steps))
+    # fifo keeps arrival order
+    reqs = requests([10] * 5, steps, cells=cells)
+    run(monkeypatch, reqs, budget_bytes=1000, max_cases=1, chunk=2)
+    assert [r.rid for r in sorted(reqs, key=lambda r: r.admit)] == [0, 1, 2, 3, 4]
+
+
+def test_fill_target(monkeypatch):
+    """With a fill target the resident active cells stay within it; a
+    request larger than the target alone runs when nothing else is
+    resident."""
+    reqs = requests([10] * 5, [4] * 5, cells=[10, 10, 10, 40, 10])
+    stats, b = run(monkeypatch, reqs, budget_bytes=1000, max_cases=8, chunk=2, policy="shortest",
+                   fill_cells=25)
+    assert max(b.resident_log) == 2
+    big = reqs[3]
+    for r in reqs:
+        if r is not big:
+            assert not (r.admit < big.finish and big.admit < r.finish)  # never resident together
+    assert all(r.error is None and r.result[3] == 4 for r in reqs) and FakeSolver.live == 0
+
+
+def test_unknown_policy_rejected():
+    with pytest.raises(ValueError):
+        S.Server(policy="lifo")
