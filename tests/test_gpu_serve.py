@@ -11,12 +11,15 @@ from hydraphase import serve as S
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_served_results_equal_alone(precision):
+@pytest.mark.parametrize("precision,policy", [(0, "fifo"), (1, "fifo"), (0, "shortest")])
+def test_served_results_equal_alone(precision, policy):
+    """Smallest-first admission with a fill target changes only when each
+    case runs and beside which others."""
     reqs = S.poisson_requests(8, rate=400.0, seed=3, nmin=4, nmax=8, steps=(3, 12), precision=precision)
     biggest = max(r.nbytes for r in reqs)
     budget = 2.5 * biggest
-    srv = S.Server(budget_bytes=budget, max_cases=16, chunk=4)
+    fill = None if policy == "fifo" else 2 * max(int(r.mesh.iblank.sum()) for r in reqs)
+    srv = S.Server(budget_bytes=budget, max_cases=16, chunk=4, policy=policy, fill_cells=fill)
     try:
         stats = srv.run(reqs)
     finally:
