@@ -748,25 +748,22 @@ int hp_set_donors(hp_ctx *c, int n, const int *recv, const int *donors, int ld, 
     c->dbufs.clear();
     c->D = Donors{};
     c->s.dflag = nullptr;
+    c->s.cnd = nullptr;
     c->ts.hS[0].dflag = nullptr;
+    c->ts.hS[0].cnd = nullptr;
     if (n == 0) return ts_upload(c->ts, c->st, c->err);
-    // the distinct donors and a per-cell donor flag (the fused step's flux
-    // leaves their cn to the gather)
+    // a per-cell donor flag, and where the fused step's flux keeps the
+    // donors' stage update for the gather
     std::vector<unsigned char> hflag(ne, 0);
     for (int r = 0; r < n; r++) {
         hflag[hid[r].x] = 1; hflag[hid[r].y] = 1; hflag[hid[r].z] = 1; hflag[hid[r].w] = 1;
     }
-    std::vector<int> hdons;
-    for (int i = 0; i < ne; i++)
-        if (hflag[i]) hdons.push_back(i);
-    const int nd = (int)hdons.size();
     void *pf = nullptr, *pd = nullptr;
     HIPCHK(c, hipMalloc(&pf, ne));
     c->dbufs.push_back(pf);
-    HIPCHK(c, hipMalloc(&pd, sizeof(int) * nd));
+    HIPCHK(c, hipMalloc(&pd, sizeof(double) * 10 * (size_t)ne));
     c->dbufs.push_back(pd);
     HIPCHK(c, hipMemcpyAsync(pf, hflag.data(), ne, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(pd, hdons.data(), sizeof(int) * nd, hipMemcpyHostToDevice, c->st));
     void *pr = nullptr, *pi = nullptr, *pw = nullptr;
     HIPCHK(c, hipMalloc(&pr, sizeof(int) * n));
     c->dbufs.push_back(pr);
@@ -782,10 +779,10 @@ int hp_set_donors(hp_ctx *c, int n, const int *recv, const int *donors, int ld, 
     c->D.recv = static_cast<const int *>(pr);
     c->D.ids = static_cast<const int4 *>(pi);
     c->D.w = static_cast<const double2 *>(pw);
-    c->D.nd = nd;
-    c->D.dons = static_cast<const int *>(pd);
     c->s.dflag = static_cast<const unsigned char *>(pf);
+    c->s.cnd = static_cast<double *>(pd);
     c->ts.hS[0].dflag = c->s.dflag;
+    c->ts.hS[0].cnd = c->s.cnd;
     if (int rc = ts_upload(c->ts, c->st, c->err)) return rc;
     HIPCHK(c, hipStreamSynchronize(c->st));
     return HP_OK;
@@ -928,6 +925,7 @@ int hp_batch_add(hp_batch *b, hp_ctx *c, int *slot)
     b->ts.hP[k] = c->P;
     b->ts.hS[k] = c->s;
     b->ts.hS[k].dflag = nullptr;  // batch steps run no donor gather
+    b->ts.hS[k].cnd = nullptr;
     b->ts.hM[k] = c->m;
     b->ts.used[k] = 1;
     if (int rc = ts_upload(b->ts, b->st, b->err)) {

@@ -56,6 +56,7 @@ struct State {
     double *scal;           // SC_N
     int *itot;              // total (step counter used by the outflow BC)
     const unsigned char *dflag;  // ne: 1 at overset donors (null without donor lists)
+    double *cnd;            // 10 x ne: the donors' stage update for the fused step's gather (with donor lists)
 };
 
 // Work of one launch.  Cells (and boundary ghosts) of every case are cut
@@ -125,11 +126,9 @@ struct Donors {
     const int *recv;      // n
     const int4 *ids;      // n, 0-based
     const double2 *w;     // 2n: (w1, w2), (w3, w4)
-    int nd;               // distinct donors
-    const int *dons;      // nd, 0-based
 };
-// eos: 0 the gather alone; fused step, 1 then the receptors' and the donors'
-// primitive recovery into s.q, 2 as 1 plus cu = cn (last stage)
+// eos: 0 the gather from cn alone; fused step, 1 the gather from cnd, then
+// the receptors' primitive recovery into s.q, 2 as 1 plus cu = cn (last stage)
 void launch_donor_gather(const Params &P, const State &s, const Donors &D, int eos, hipStream_t st);
 
 }  // namespace hydra

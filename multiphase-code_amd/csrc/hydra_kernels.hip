@@ -1174,8 +1174,8 @@ __device__ __forceinline__ void source_and_update(const State &s, const Mesh &m,
 // thread per cell recovers the primitives into the other primitive buffer
 // q2 (the next stage's q) and stores the rewritten conserved variables in
 // cn, and in cu as well at the end of the step (`end`).  An overset donor
-// stores its stage update unrewritten: the donor gather reads it, and the
-// donors' own rewrite follows the gather (k_donor_eos).
+// also keeps its stage update, unrewritten, in cnd: that is what the donor
+// gather after this launch interpolates.
 __device__ __forceinline__ void eos_staged(const Params &P, const State &s, int i, const double *c_lds, bool end)
 {
     double c[10], v[Q_N], nv[10];
@@ -1184,10 +1184,8 @@ __device__ __forceinline__ void eos_staged(const Params &P, const State &s, int 
     const int ne = s.ne;
     const size_t nt = s.nt;
     for (int k = 0; k < Q_N; k++) s.q2[(size_t)k * nt + i] = v[k];
-    if (s.dflag && s.dflag[i]) {
-        for (int k = 0; k < 10; k++) s.cn[(size_t)k * ne + i] = c[k];
-        return;
-    }
+    if (s.dflag && s.dflag[i])
+        for (int k = 0; k < 10; k++) s.cnd[(size_t)k * ne + i] = c[k];
     for (int k = 0; k < 10; k++) s.cn[(size_t)k * ne + i] = nv[k];
     if (end)
         for (int k = 0; k < 10; k++) s.cu[(size_t)k * ne + i] = nv[k];
@@ -2356,8 +2354,11 @@ __global__ void __launch_bounds__(256) k_donor_gather(Params P, State s, Donors 
     const int4 d = D.ids[r];
     const double2 wa = D.w[2 * r], wb = D.w[2 * r + 1];
     const int i = D.recv[r], ne = s.ne;
+    // fused step: the donors' stage update as the flux launch kept it (cnd);
+    // their cn already holds the rewrite of their primitive recovery
+    const double *src = eos ? s.cnd : s.cn;
     for (int k = 0; k < 10; k++) {
-        const double *c = s.cn + (size_t)k * ne;
+        const double *c = src + (size_t)k * ne;
         s.cn[(size_t)k * ne + i] = wa.x * c[d.x] + wa.y * c[d.y] + wb.x * c[d.z] + wb.y * c[d.w];
     }
     // fused step: the receptor's primitive recovery into s.q (the next
@@ -2365,20 +2366,9 @@ __global__ void __launch_bounds__(256) k_donor_gather(Params P, State s, Donors 
     if (eos) eos_cell(P, s.cn, ne, i, s.q, s.nt, nullptr, s.dl, eos == 2 ? s.cu : nullptr);
 }
 
-// fused step, after the gather: the donors' primitive recovery, which the
-// flux epilogue left out of cn so that the gather read the stage update
-__global__ void __launch_bounds__(256) k_donor_eos(Params P, State s, Donors D, int end)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= D.nd) return;
-    eos_cell(P, s.cn, s.ne, D.dons[r], s.q, s.nt, nullptr, s.dl, end ? s.cu : nullptr);
-}
-
 void launch_donor_gather(const Params &P, const State &s, const Donors &D, int eos, hipStream_t st)
 {
     if (D.n > 0) hipLaunchKernelGGL(k_donor_gather, dim3((D.n + 255) / 256), dim3(256), 0, st, P, s, D, eos);
-    if (eos && D.nd > 0)
-        hipLaunchKernelGGL(k_donor_eos, dim3((D.nd + 255) / 256), dim3(256), 0, st, P, s, D, eos == 2 ? 1 : 0);
 }
 
 }  // namespace hydra
