@@ -69,3 +69,24 @@ def test_donor_lists_rejected_when_bad(case):
     with pytest.raises(HydraError):
         s.set_donors(recv, bad, w)
     s.close()
+
+
+def test_cleared_donor_lists_step_as_none(case):
+    """hp_set_donors with no receptors drops the lists (and the fused step's
+    donor flags and staging): the case then steps exactly as one that never
+    had donor lists."""
+    m, p, cu, gp, recv, donors, w = case
+    s = Solver(m, p)
+    s.upload(cu, gp, 0.0, 0)
+    s.set_donors(recv, donors, w)
+    s.set_donors(np.zeros(0, dtype=np.int32), np.zeros((0, 4), dtype=np.int32), np.zeros((0, 4)))
+    s.step(2)
+    got = s.download()
+    s.close()
+    s = Solver(m, p)
+    s.upload(cu, gp, 0.0, 0)
+    s.step(2)
+    ref = s.download()
+    s.close()
+    assert got[3] == ref[3] == 2 and got[2] == ref[2]
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
