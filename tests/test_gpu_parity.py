@@ -265,3 +265,33 @@ def test_large_box_steps_bitwise():
     assert tot == tot_o == 3 and t == t_o
     assert_same("cu", cu, cu_o)
     assert_same("ghost prims", gp, gp_o)
+
+
+def test_bench_case_step_bitwise():
+    """The benchmark's own workload at full size: the 96x96x90 box cut at
+    r = 0.15 plus the near-body annulus, coupled by donor lists (5,230,160
+    tets, 224,877 receptors), 8 water spheres, viscous, FP64.  One fused
+    step (with the donor gather after every stage) equals the oracle's bit
+    for bit."""
+    import bench
+    m, p, cu0, gp0, (recv, ids, w) = bench.build_case(96, 96, 90, overset=True)
+    print(f"bench case: {m.neles} tets, {m.nghosts} ghosts, {recv.size} receptors", flush=True)
+    s = solver_cls()(m, p)
+    try:
+        s.upload(cu0, gp0, 0.0, 0)
+        s.set_donors(recv, ids, w)
+        s.apply_donors()
+        s.step(1)
+        cu, gp, t, tot = s.download()
+    finally:
+        s.close()
+    o = O.Oracle(m, p)
+    o.set_state(cu0, gp0, 0.0, 0)
+    o.set_donors(recv, ids, w)
+    o.call("donor_gather")
+    o.run(1)
+    cur, gpr, tr, totr = o.get_state()
+    assert tot == totr == 1 and t == tr
+    assert np.isfinite(cu).all()
+    assert_same("cu", cu, cur)
+    assert_same("ghost prims", gp, gpr)
