@@ -295,3 +295,25 @@ def test_bench_case_step_bitwise():
     assert np.isfinite(cu).all()
     assert_same("cu", cu, cur)
     assert_same("ghost prims", gp, gpr)
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 1, 1), (3, 2, 1)])
+def test_meshes_below_one_tile_bitwise(dims):
+    """Meshes of 6, 12 and 36 tets, one partial tile (most lanes of the
+    block idle, every cell at a boundary): three steps, fused and per
+    routine, equal the oracle's bit for bit."""
+    m = M.box_mesh(*dims, jitter=0.1, seed=3)
+    p = M.water_air_params(cfl=0.5, viscous_gas=True)
+    cu0, gp0 = M.sphere_state(m, p, [(0.0, 0.0, 0.0, 0.3)])
+    for fused in (True, False):
+        o = O.Oracle(m, p)
+        o.set_state(cu0, gp0, 0.0, 0)
+        s = solver_cls()(m, p)
+        try:
+            s.upload(cu0, gp0, 0.0, 0)
+            (cu, gp, t, tot), (cur, gpr, tr, totr) = run_pair(m, p, o, s, 3, fused=fused)
+        finally:
+            s.close()
+        assert tot == totr == 3 and t == tr
+        assert_same(f"cu fused={fused}", cu, cur)
+        assert_same(f"ghost prims fused={fused}", gp, gpr)
