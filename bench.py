@@ -10,7 +10,10 @@ barycentric donor lists (two fringe layers each side, applied after every
 RK stage).  Gas viscosity on (air 1.8e-5 Pa s via Sutherland, water 1e-3
 Pa s), a train of 8 water spheres (radii 0.03-0.08, seeded) in an air
 stream at Mach 0.3, CFL 0.5.  FP64 throughout (--precision 1: FP32 bulk).
---single-mesh runs the box alone with the cylinder as a wall hole.
+--single-mesh runs the box alone with the cylinder as a wall hole.  The
+box's cells are numbered in y-z slabs of 8 x 8 hex layers (x fastest within
+a slab; hydraphase.mesh.slab_order) so that face neighbours sit close in
+memory; per-cell results do not depend on the numbering.
 
 A step is one full time step of the reference (main.F90:515-625): primitive,
 CFL reduction, JST smoothing, viscous fluxes and four RK stages of
@@ -57,19 +60,24 @@ FLUX_BYTES_PER_CELL = 112 + 264 + 8 + 96 + 96 + 24 + 16 + 8 + 40 + 64 + 80 + 80 
 FLUX_BYTES_PER_CELL_F32 = FLUX_BYTES_PER_CELL - 132 - 4 + 16 - 48
 
 
-def build_case(nx, ny, nz, seed=1, precision=0, overset=True):
+def build_case(nx, ny, nz, seed=1, precision=0, overset=True, slabs=(8, 8)):
     """(mesh, params, cu, gp, donors) of the single large case; donors is
-    (recv, ids, weights) or None for the single-mesh form."""
+    (recv, ids, weights) or None for the single-mesh form.  The box's cells
+    are numbered in y-z slabs of `slabs` (hydraphase.mesh.slab_order) for
+    gather locality (None: as generated); per-cell results do not depend
+    on the numbering."""
     from hydraphase import mesh as M
     from hydraphase import overset as OV
     hole = (0.0, 0.0, 0.1)
     donors = None
     if overset:
         m, recv, ids, w = OV.overset_mesh(nx, ny, nz, nr=10, ntheta=112, r_body=0.1, r_outer=0.25,
-                                          r_cut=0.15, fringe=2, jitter=0.1, seed=seed)
+                                          r_cut=0.15, fringe=2, jitter=0.1, seed=seed, slabs=slabs)
         donors = (recv, ids, w)
     else:
         m = M.box_mesh(nx, ny, nz, jitter=0.1, seed=seed, hole=hole)
+        if slabs is not None:
+            m = m.permuted(M.slab_order(m, ny, nz, by=slabs[0], bz=slabs[1]))
     p = M.water_air_params(cfl=0.5, viscous_gas=True)
     p["precision"] = precision
     spheres = M.sphere_train(seed, n=8, rmin=0.03, rmax=0.08, avoid=(0.0, 0.0, 0.25 if overset else 0.1))
@@ -407,7 +415,8 @@ def main():
                                 "annulus 10x112 hexes x6 around the r=0.1 wall, overset donor lists"
                                 if donors is not None else
                                 f"box {args.n[0]}x{args.n[1]}x{args.n[2]} hexes x6 tets, cylinder wall hole r=0.1")
-                               + ", 8 water spheres, viscous, " + ("FP64" if args.precision == 0 else "FP32 bulk"),
+                               + ", box cells in 8x8 y-z slabs, 8 water spheres, viscous, "
+                               + ("FP64" if args.precision == 0 else "FP32 bulk"),
                    "tets": m.neles, "active_cells": active, "ghosts": m.nghosts,
                    "receptors": 0 if donors is None else int(donors[0].size), "stages_per_step": 4},
         "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -247,6 +247,32 @@ def morton_order(mesh: Mesh, bits: int = 10) -> np.ndarray:
     return np.argsort(code, kind="stable")
 
 
+def slab_order(mesh: Mesh, ny: int, nz: int, *, by: int = 8, bz: int = 8, lo=(-0.5, -0.5, -0.5),
+               hi=(0.5, 0.5, 0.5), cells=None) -> np.ndarray:
+    """Cell permutation for gather locality on a mesh generated from an
+    ny x nz (in y, z) structured box with x fastest (box_mesh, box_tets):
+    cells sorted by (z slab of bz layers, y slab of by rows, original
+    index).  A y or z face neighbour then sits about nx*by or nx*by*bz
+    hexes away instead of nx or nx*ny, except across slab boundaries.
+    ``cells=(start, stop)`` reorders only that range (another mesh's cells
+    appended after it keep their place).  Per-cell results do not depend on
+    the numbering; the permutation is for Mesh.permuted (new cell j is old
+    cell perm[j])."""
+    ne = mesh.neles
+    start, stop = (0, ne) if cells is None else cells
+    cy = mesh.y[mesh.nod].mean(axis=0)
+    cz = mesh.z[mesh.nod].mean(axis=0)
+    iy = np.clip(np.floor((cy - lo[1]) / (hi[1] - lo[1]) * ny), 0, ny - 1).astype(np.int64)
+    iz = np.clip(np.floor((cz - lo[2]) / (hi[2] - lo[2]) * nz), 0, nz - 1).astype(np.int64)
+    idx = np.arange(ne)
+    inside = (idx >= start) & (idx < stop)
+    # the reordered range keeps its place among the others
+    seg = np.where(idx < start, 0, np.where(inside, 1, 2))
+    zb = np.where(inside, iz // bz, 0)
+    yb = np.where(inside, iy // by, 0)
+    return np.lexsort((idx, yb, zb, seg))
+
+
 # --------------------------------------------------------------------------
 # flow parameters and initial states
 # --------------------------------------------------------------------------

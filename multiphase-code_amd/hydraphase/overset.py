@@ -103,19 +103,23 @@ def read_donors(path: str):
 
 def overset_mesh(nx: int = 96, ny: int = 96, nz: int = 90, *, nr: int = 10, ntheta: int = 112,
                  r_body: float = 0.1, r_outer: float = 0.25, r_cut: float = 0.15, fringe: int = 2,
-                 jitter: float = 0.1, seed: int = 1):
+                 jitter: float = 0.1, seed: int = 1, slabs=None):
     """Background box (Kuhn-split, jittered) with the cells of centroid
     radius < r_cut removed, plus the near-body annulus r_body..r_outer
     (nr x ntheta x nz hexes x 6 tets), in one mesh: background cells first.
     Receptors (iblank 0, filled from the other mesh) are the background
     cells within `fringe` background spacings outside r_cut and the annulus
-    cells of the outer `fringe` radial layers.  Returns (mesh, recv,
-    donors, weights) with 0-based ids."""
+    cells of the outer `fringe` radial layers.  ``slabs=(by, bz)`` numbers
+    the background cells in y-z slabs (mesh.slab_order) for gather
+    locality; the annulus keeps its order.  Returns (mesh, recv, donors,
+    weights) with 0-based ids."""
     bg = M.box_tets(nx, ny, nz, jitter=jitter, seed=seed,
                     keep=lambda cx, cy, cz: cx * cx + cy * cy >= r_cut * r_cut)
     an = M.annulus_mesh(nr, ntheta, nz, r0=r_body, r1=r_outer)
     m = M.combine([bg, an], shape=(nx, ny, nz))
     nbg = bg[3].shape[0]
+    if slabs is not None:
+        m = m.permuted(M.slab_order(m, ny, nz, by=slabs[0], bz=slabs[1], cells=(0, nbg)))
     c = centroids(m)
     rad = np.hypot(c[:, 0], c[:, 1])
     h = 1.0 / nx

@@ -94,3 +94,31 @@ def test_oracle_gather_reproduces_linear_conserved_fields(small):
     o.call("donor_gather")
     for k, (a, b) in enumerate(lin):
         assert np.abs(cn[k, recv] - (a + c[recv] @ b)).max() < 1e-12 * (abs(a) + np.abs(b).sum())
+
+
+def test_slab_numbering_keeps_the_case():
+    """Numbering the background in y-z slabs gives the same cells (same
+    centroids), the same receptors with the same donors and weights up to
+    the renumbering, the annulus in place, and brings y/z face neighbours
+    closer."""
+    m0, r0, d0, w0 = OV.overset_mesh(16, 16, 16, nr=5, ntheta=24, r_outer=0.35, fringe=1)
+    m1, r1, d1, w1 = OV.overset_mesh(16, 16, 16, nr=5, ntheta=24, r_outer=0.35, fringe=1, slabs=(4, 4))
+    c0, c1 = OV.centroids(m0), OV.centroids(m1)
+    k0, k1 = np.lexsort(c0.T[::-1]), np.lexsort(c1.T[::-1])
+    assert np.array_equal(c0[k0], c1[k1])
+    new_of_old = np.empty(m0.neles, dtype=np.int64)
+    new_of_old[k0] = k1
+    pos1 = {int(r): k for k, r in enumerate(r1)}
+    assert sorted(pos1) == sorted(int(x) for x in new_of_old[r0])
+    for k in range(r0.size):
+        j = pos1[int(new_of_old[r0[k]])]
+        assert np.array_equal(new_of_old[d0[k]], d1[j]) and np.array_equal(w0[k], w1[j])
+    nbg = m0.neles - 5 * 24 * 16 * 6  # the annulus (nr x ntheta x nz hexes x 6) comes last
+    assert np.array_equal(new_of_old[nbg:], np.arange(nbg, m0.neles))
+
+    def spread(m):
+        nc = np.asarray(m.nc)
+        i = np.arange(m.neles)
+        d = np.concatenate([np.abs(nc[f] - i)[nc[f] < m.neles] for f in range(4)])
+        return np.percentile(d, 90)
+    assert spread(m1) < spread(m0)
