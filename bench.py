@@ -259,13 +259,15 @@ def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
     specs = HB.ensemble_specs(64, seed=1, steps=(steps, steps))
     t0 = time.time()
     tuning = HB.tune(specs, precision=1, log=log)
-    log(f"# tuned in {time.time() - t0:.1f} s: morton={tuning['morton']} grid={tuning['grid']}")
+    log(f"# tuned in {time.time() - t0:.1f} s: morton={tuning['morton']} slabs={tuning['slabs']} "
+        f"grid={tuning['grid']}")
     res = HB.run_ensemble(specs, precision=1, tuning=tuning, graph=True)
     batched = res["updates"] / (res["ms"] * 1e-3)
-    # plain one-case FP64 path: each case alone, baseline_steps steps after 2 warm-up steps
+    # plain one-case FP64 path: each case alone (in the same cell order),
+    # baseline_steps steps after 2 warm-up steps
     upd, ms = 0, 0.0
     for sp in specs:
-        m, p, cu, gp, _ = HB.build_case(sp, 0)
+        m, p, cu, gp, _ = HB.build_case(sp, 0, tuning["morton"], 2, tuning["slabs"])
         s = Solver(m, p)
         s.upload(cu, gp, 0.0, 0)
         s.step(2)
@@ -294,7 +296,7 @@ def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
                         f"R 0.05-0.25, half viscous, {steps} steps each",
             "updates_per_s": batched, "ms_per_step": res["ms"] / steps,
             "plain_fp64_one_case_updates_per_s": plain, "speedup_vs_plain_fp64": batched / plain,
-            "tuning": {"morton": tuning["morton"], "grid": list(tuning["grid"])},
+            "tuning": {"morton": tuning["morton"], "slabs": tuning["slabs"], "grid": list(tuning["grid"])},
             "mixture_rel_l2_vs_fp64_sample": errs, "sample_cases": sample,
             "cases_unhealthy_at_end": int(sum(1 for h in res["health"] if h))}
 

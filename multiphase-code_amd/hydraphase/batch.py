@@ -133,18 +133,20 @@ def ensemble_specs(n_cases: int = 64, seed: int = 1, nmin: int = 8, nmax: int = 
             for k in range(n_cases)]
 
 
-def build_case(spec: CaseSpec, precision: int = 0, morton: bool = False, myflux: int = 2):
+def build_case(spec: CaseSpec, precision: int = 0, morton: bool = False, myflux: int = 2,
+               slabs: bool = False):
     """(mesh, params, cu, gp, perm) of one ensemble member.  morton: cells
-    sorted along a Z-order curve (new cell j is generated cell perm[j]; the
-    results per cell are the same, only the memory order changes); perm is
-    None otherwise."""
+    sorted along a Z-order curve; slabs: cells numbered in y-z slabs of 8 x
+    8 hex layers (mesh.slab_order).  New cell j is generated cell perm[j];
+    the results per cell are the same, only the memory order changes; perm
+    is None for the generated order."""
     m = M.box_mesh(spec.n, spec.n, spec.n, jitter=0.1, seed=spec.seed)
     p = M.water_air_params(mach=spec.mach, cfl=0.5, viscous_gas=spec.viscous, myflux=myflux)
     p["precision"] = precision
     cu, gp = M.sphere_state(m, p, [(0.0, 0.0, 0.0, spec.radius)])
     perm = None
-    if morton:
-        perm = M.morton_order(m)
+    if morton or slabs:
+        perm = M.morton_order(m) if morton else M.slab_order(m, spec.n, spec.n)
         m = m.permuted(perm)
         cu = np.ascontiguousarray(cu[:, perm])  # ghost order (and gp) is kept
     return m, p, cu, gp, perm
@@ -174,12 +176,14 @@ def tune(specs, precision: int = 1, steps: int = 3, myflux: int = 2, log=None) -
     Builds a scratch copy of the ensemble (so the real cases do not move),
     then walks the choices one family at a time (coordinate descent): cell
     order, then cell-kernel blocks, face-kernel blocks and ghost-kernel
-    blocks, each from GRID_CHOICES (0 = one block per tile).  Returns
-    {"morton", "grid": (cell, quad, ghost), "ms_per_step", "trials"}."""
+    blocks, each from GRID_CHOICES (0 = one block per tile).  The cell
+    orders tried: as generated, Z order (morton) and y-z slabs (slabs).
+    Returns {"morton", "slabs", "grid": (cell, quad, ghost), "ms_per_step",
+    "trials"}."""
     trials = []
 
-    def build(morton):
-        cases = [build_case(s, precision, morton, myflux) for s in specs]
+    def build(order):
+        cases = [build_case(s, precision, order["morton"], myflux, order["slabs"]) for s in specs]
         ms = [c[0] for c in cases]
         b = Batch(*capacity(ms))
         sols = []
@@ -191,18 +195,20 @@ def tune(specs, precision: int = 1, steps: int = 3, myflux: int = 2, log=None) -
         return b, sols
 
     best = None
-    for morton in (False, True):
-        b, sols = build(morton)
+    for order in ({"morton": False, "slabs": False}, {"morton": True, "slabs": False},
+                  {"morton": False, "slabs": True}):
+        b, sols = build(order)
         t = _time_steps(b, steps)
-        trials.append(({"morton": morton, "grid": (0, 0, 0)}, t))
+        trials.append((dict(order, grid=(0, 0, 0)), t))
         if log:
-            log(f"tune: morton={morton} grid=(0,0,0) {t:.3f} ms/step")
+            log(f"tune: order={order} grid=(0,0,0) {t:.3f} ms/step")
         if best is None or t < best[1]:
-            best = ({"morton": morton, "grid": (0, 0, 0)}, t)
+            best = (dict(order, grid=(0, 0, 0)), t)
         b.close()
         for s in sols:
             s.close()
-    b, sols = build(best[0]["morton"])
+    order = {"morton": best[0]["morton"], "slabs": best[0]["slabs"]}
+    b, sols = build(order)
     grid = list(best[0]["grid"])
     best_t = best[1]
     for fam in range(3):
@@ -213,7 +219,7 @@ def tune(specs, precision: int = 1, steps: int = 3, myflux: int = 2, log=None) -
             trial[fam] = g
             b.set_grid(*trial)
             t = _time_steps(b, steps)
-            trials.append(({"morton": best[0]["morton"], "grid": tuple(trial)}, t))
+            trials.append((dict(order, grid=tuple(trial)), t))
             if log:
                 log(f"tune: grid={tuple(trial)} {t:.3f} ms/step")
             if t < best_t:
@@ -221,7 +227,7 @@ def tune(specs, precision: int = 1, steps: int = 3, myflux: int = 2, log=None) -
     b.close()
     for s in sols:
         s.close()
-    return {"morton": best[0]["morton"], "grid": tuple(grid), "ms_per_step": best_t, "trials": trials}
+    return dict(order, grid=tuple(grid), ms_per_step=best_t, trials=trials)
 
 
 def run_ensemble(specs, precision: int = 1, tuning: dict | None = None, graph: bool = True,
@@ -231,7 +237,7 @@ def run_ensemble(specs, precision: int = 1, tuning: dict | None = None, graph: b
     hp_health status when they left, and the timing: {"ms": total,
     "updates": cell-stage updates, "states": [...], "health": [...]}."""
     tuning = tuning or {"morton": False, "grid": (0, 0, 0)}
-    cases = [build_case(s, precision, tuning["morton"], myflux) for s in specs]
+    cases = [build_case(s, precision, tuning["morton"], myflux, tuning.get("slabs", False)) for s in specs]
     b = Batch(*capacity([c[0] for c in cases]))
     b.set_grid(*tuning["grid"])
     sols, slots = [], []
