@@ -23,7 +23,7 @@ stages) per second with the state resident in HBM.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
        python bench.py summarize-counters FETCH.csv WRITE.csv CALIB_MB
-       python bench.py summarize-trace results.db
+       python bench.py summarize-trace results.db [KERNEL-FOR-A-PER-GRID-SPLIT]
        python bench.py summarize-instructions counter_collection.csv
 
 roofline.traffic is the HBM bytes of one k_flux launch measured with
@@ -237,14 +237,23 @@ def summarize_instructions(sq_csv):
               f"{c['SQ_INSTS_VMEM_RD'] / n:.3e} | {c['SQ_INSTS_LDS'] / n:.3e} | {act:.2f} |")
 
 
-def summarize_trace(db_path, top=20):
-    """Per-kernel totals from a rocprofv3 --kernel-trace --stats database."""
+def summarize_trace(db_path, top=20, by_grid=None):
+    """Per-kernel totals from a rocprofv3 --kernel-trace --stats database;
+    with by_grid (a kernel name substring), that kernel's launches split by
+    grid size as well (a run that launches it for cases of several sizes)."""
     import sqlite3
     cur = sqlite3.connect(db_path).cursor()
     print("| kernel | calls | total us | avg us | % |\n|---|---|---|---|---|")
     for r in cur.execute(f"select * from top_kernels limit {int(top)}"):
         name = r[0].replace("|", "/")
         print(f"| `{name}` | {r[1]} | {r[2]:.1f} | {r[3]:.1f} | {r[4]:.2f} |")
+    if by_grid:
+        print(f"\n`{by_grid}` by grid size (work-items), largest first:\n")
+        print("| grid | calls | avg us |\n|---|---|---|")
+        q = ("select grid_x, count(*), avg(end - start) from kernels where name like ? "
+             "group by grid_x order by grid_x desc limit 8")
+        for g, n, d in cur.execute(q, (f"%{by_grid}%",)):
+            print(f"| {g} | {n} | {d / 1e3:.1f} |")
 
 
 def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
@@ -336,7 +345,7 @@ def main():
         summarize_instructions(sys.argv[2])
         return
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-trace":
-        summarize_trace(sys.argv[2])
+        summarize_trace(sys.argv[2], by_grid=sys.argv[3] if len(sys.argv) > 3 else None)
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
