@@ -2142,15 +2142,16 @@ __global__ void __launch_bounds__(4 * TILE) k_viscous(Batch B)
     }
 }
 
-// Unconstrained, the FP32-bulk flux compiles to 90 VGPRs and five of its
-// blocks fit on a CU.  A 48 KB static LDS reservation holds residency at
-// three blocks, and the compiler then schedules with up to 132 VGPRs (more
-// gathers in flight per wave): measured at 4.8M cells this is the fastest
-// (2.07 ms, against 2.49 ms unconstrained and 2.84 ms with a reservation
-// allowing four; the same 48 KB requested as dynamic LDS at launch, which
-// the compiler does not see, measured 3.46 ms).
-// The fused step's staging of a tile's new conserved variables (eos_staged)
-// uses the first 10 x TILE doubles of the same reservation.
+// The FP32-bulk flux runs fastest held to three blocks per CU by a 48 KB
+// static LDS reservation: the compiler then schedules with more registers
+// (more gathers in flight per wave).  At 4.8M cells: 2.22 ms with it (150
+// VGPRs), 3.01 ms without (109 VGPRs, four waves per SIMD); before the
+// fused primitive recovery 2.07 ms against 2.49 unconstrained and 2.84 with
+// a reservation allowing four blocks; the same 48 KB requested as dynamic
+// LDS at launch, which the compiler does not see, measured 3.46 ms.  The
+// FP64 flux is the other way round: held to three or two blocks it slows
+// down.  The fused step's staging of a tile's new conserved variables
+// (eos_staged) uses the first 10 x TILE doubles of the reservation.
 constexpr int FLUX_F32_LDS_CAP = 48 * 1024;
 constexpr int FLUX_STAGE_DOUBLES = 10 * TILE;
 static_assert(FLUX_STAGE_DOUBLES * 8 <= FLUX_F32_LDS_CAP, "staging fits the FP32 reservation");
