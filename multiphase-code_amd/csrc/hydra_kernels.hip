@@ -508,13 +508,17 @@ __device__ __forceinline__ void smooth1_cell(const State &s, const Mesh &m, int 
     double a2 = anupp + num / (den + F1E6);
     num = switch_sum(s.cn + (size_t)9 * ne, i, nb, &den);
     a2 = a2 + num / (den + F1E6);
-    s.anu[i] = a1;
-    s.anu[(size_t)ne + i] = a2;
+    // every store after every load: the stores could alias cn for all the
+    // compiler knows, and the switch sums' loads of cn 0, 4, 5, 9 are reused
+    double d2[10];
     for (int k = 0; k < 10; k++) {
         const double *c = s.cn + (size_t)k * ne;
         const double ci = c[i];
-        s.d2q[(size_t)k * ne + i] = c[nb[0]] - ci + c[nb[1]] - ci + c[nb[2]] - ci + c[nb[3]] - ci;
+        d2[k] = c[nb[0]] - ci + c[nb[1]] - ci + c[nb[2]] - ci + c[nb[3]] - ci;
     }
+    s.anu[i] = a1;
+    s.anu[(size_t)ne + i] = a2;
+    for (int k = 0; k < 10; k++) s.d2q[(size_t)k * ne + i] = d2[k];
 }
 
 __device__ __forceinline__ void smooth2_cell(const Params &P, const State &s, const Mesh &m, int i)
