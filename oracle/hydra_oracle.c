@@ -1335,6 +1335,57 @@ static void riemann_zien(const hyo_params *P, const face_state *L, const face_st
 /* face loop of sr_hllcbatten.F90:59-556 (myflux=2) or sr_hllczien.F90:62-502
  * (myflux=1); the two differ in the Riemann solver and in whether partition
  * ghosts are reconstructed (none exist on one rank). */
+/* One face's left/right states after the second-order reconstruction and
+ * before the phase clipping (sr_hllcbatten.F90:135-224 with the
+ * reconstruction condition of :169,193; sr_hllczien.F90:138-217, which
+ * reconstructs across interior faces only).  Returns the neighbour. */
+static int face_states(const hyo_case *c, int i, int f, double *xf_out, face_state *L, face_state *R)
+{
+    const hyo_params *P = &c->p;
+    const int ne = c->neles, ntot_bc = c->neles + c->nghosts;
+    const int zien = (P->myflux == 1);
+    int a, b, d;
+    flux_face_nodes(c, i, f, &a, &b, &d);
+    int neigh = NC(i, f);
+    double xf = (c->x[a] + c->x[b] + c->x[d]) * 0.3333;
+    double yf = (c->y[a] + c->y[b] + c->y[d]) * 0.3333;
+    double zf = (c->z[a] + c->z[b] + c->z[d]) * 0.3333;
+    xf_out[0] = xf; xf_out[1] = yf; xf_out[2] = zf;
+    load_state(c, i, L);
+    load_state(c, neigh, R);
+    int recon = zien ? (neigh < ne) : (neigh < ne || neigh >= ntot_bc);
+    if (P->myorder == 2 && recon) {
+        double phimin = dmin2(c->phi[i], c->phi[neigh]);
+        double xf0 = xf - c->xc[i], yf0 = yf - c->yc[i], zf0 = zf - c->zc[i];
+        double xfn = xf - c->xc[neigh], yfn = yf - c->yc[neigh], zfn = zf - c->zc[neigh];
+        L->phig = L->phig + phimin * GRD(HG_PHIG, i, xf0, yf0, zf0);
+        L->ug = L->ug + phimin * GRD(HG_UG, i, xf0, yf0, zf0);
+        L->vg = L->vg + phimin * GRD(HG_VG, i, xf0, yf0, zf0);
+        L->wg = L->wg + phimin * GRD(HG_WG, i, xf0, yf0, zf0);
+        L->tg = L->tg + phimin * GRD(HG_TG, i, xf0, yf0, zf0);
+        L->pg = L->pg + phimin * GRD(HG_PG, i, xf0, yf0, zf0);
+        R->phig = R->phig + phimin * GRD(HG_PHIG, neigh, xfn, yfn, zfn);
+        R->ug = R->ug + phimin * GRD(HG_UG, neigh, xfn, yfn, zfn);
+        R->vg = R->vg + phimin * GRD(HG_VG, neigh, xfn, yfn, zfn);
+        R->wg = R->wg + phimin * GRD(HG_WG, neigh, xfn, yfn, zfn);
+        R->tg = R->tg + phimin * GRD(HG_TG, neigh, xfn, yfn, zfn);
+        R->pg = R->pg + phimin * GRD(HG_PG, neigh, xfn, yfn, zfn);
+        L->up = L->up + phimin * GRD(HG_UP, i, xf0, yf0, zf0);
+        L->vp = L->vp + phimin * GRD(HG_VP, i, xf0, yf0, zf0);
+        L->wp = L->wp + phimin * GRD(HG_WP, i, xf0, yf0, zf0);
+        L->tp = L->tp + phimin * GRD(HG_TP, i, xf0, yf0, zf0);
+        L->pp = L->pp + phimin * GRD(HG_PP, i, xf0, yf0, zf0);
+        R->up = R->up + phimin * GRD(HG_UP, neigh, xfn, yfn, zfn);
+        R->vp = R->vp + phimin * GRD(HG_VP, neigh, xfn, yfn, zfn);
+        R->wp = R->wp + phimin * GRD(HG_WP, neigh, xfn, yfn, zfn);
+        R->tp = R->tp + phimin * GRD(HG_TP, neigh, xfn, yfn, zfn);
+        R->pp = R->pp + phimin * GRD(HG_PP, neigh, xfn, yfn, zfn);
+    }
+    L->phip = 1.0 - L->phig;
+    R->phip = 1.0 - R->phig;
+    return neigh;
+}
+
 void hyo_face_flux(hyo_case *c)
 {
     const hyo_params *P = &c->p;
@@ -1345,50 +1396,14 @@ void hyo_face_flux(hyo_case *c)
         for (int k = 0; k < NEQ; k++) RHS(i, k) = 0.0;
         double F[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int f = 0; f < 4; f++) {
-            int a, b, d;
-            flux_face_nodes(c, i, f, &a, &b, &d);
-            int neigh = NC(i, f);
+            double xf[3];
+            face_state L, R;
+            int neigh = face_states(c, i, f, xf, &L, &R);
             double scx = SC(i, f, 0), scy = SC(i, f, 1), scz = SC(i, f, 2);
-            double xf = (c->x[a] + c->x[b] + c->x[d]) * 0.3333;
-            double yf = (c->y[a] + c->y[b] + c->y[d]) * 0.3333;
-            double zf = (c->z[a] + c->z[b] + c->z[d]) * 0.3333;
             double area = sqrt(scx * scx + scy * scy + scz * scz);
             double enx = scx / area, eny = scy / area, enz = scz / area;
             int nt = 0;
             if (neigh >= ne && neigh < ntot_bc) nt = c->gtype[neigh - ne];
-            face_state L, R;
-            load_state(c, i, &L);
-            load_state(c, neigh, &R);
-            int recon = zien ? (neigh < ne) : (neigh < ne || neigh >= ntot_bc);
-            if (P->myorder == 2 && recon) {
-                double phimin = dmin2(c->phi[i], c->phi[neigh]);
-                double xf0 = xf - c->xc[i], yf0 = yf - c->yc[i], zf0 = zf - c->zc[i];
-                double xfn = xf - c->xc[neigh], yfn = yf - c->yc[neigh], zfn = zf - c->zc[neigh];
-                L.phig = L.phig + phimin * GRD(HG_PHIG, i, xf0, yf0, zf0);
-                L.ug = L.ug + phimin * GRD(HG_UG, i, xf0, yf0, zf0);
-                L.vg = L.vg + phimin * GRD(HG_VG, i, xf0, yf0, zf0);
-                L.wg = L.wg + phimin * GRD(HG_WG, i, xf0, yf0, zf0);
-                L.tg = L.tg + phimin * GRD(HG_TG, i, xf0, yf0, zf0);
-                L.pg = L.pg + phimin * GRD(HG_PG, i, xf0, yf0, zf0);
-                R.phig = R.phig + phimin * GRD(HG_PHIG, neigh, xfn, yfn, zfn);
-                R.ug = R.ug + phimin * GRD(HG_UG, neigh, xfn, yfn, zfn);
-                R.vg = R.vg + phimin * GRD(HG_VG, neigh, xfn, yfn, zfn);
-                R.wg = R.wg + phimin * GRD(HG_WG, neigh, xfn, yfn, zfn);
-                R.tg = R.tg + phimin * GRD(HG_TG, neigh, xfn, yfn, zfn);
-                R.pg = R.pg + phimin * GRD(HG_PG, neigh, xfn, yfn, zfn);
-                L.up = L.up + phimin * GRD(HG_UP, i, xf0, yf0, zf0);
-                L.vp = L.vp + phimin * GRD(HG_VP, i, xf0, yf0, zf0);
-                L.wp = L.wp + phimin * GRD(HG_WP, i, xf0, yf0, zf0);
-                L.tp = L.tp + phimin * GRD(HG_TP, i, xf0, yf0, zf0);
-                L.pp = L.pp + phimin * GRD(HG_PP, i, xf0, yf0, zf0);
-                R.up = R.up + phimin * GRD(HG_UP, neigh, xfn, yfn, zfn);
-                R.vp = R.vp + phimin * GRD(HG_VP, neigh, xfn, yfn, zfn);
-                R.wp = R.wp + phimin * GRD(HG_WP, neigh, xfn, yfn, zfn);
-                R.tp = R.tp + phimin * GRD(HG_TP, neigh, xfn, yfn, zfn);
-                R.pp = R.pp + phimin * GRD(HG_PP, neigh, xfn, yfn, zfn);
-            }
-            L.phip = 1.0 - L.phig;
-            R.phip = 1.0 - R.phig;
             clip_blend(P, &L);
             clip_blend(P, &R);
             if (nt == 5) {
@@ -1407,6 +1422,28 @@ void hyo_face_flux(hyo_case *c)
             }
         }
     }
+}
+
+/* The face states hyo_face_flux works from, for tests: for every interior
+ * cell i and face f, out[((i*4+f)*2+side)*HF_N + field] with side 0 = left
+ * (cell i), 1 = right (its neighbour), fields in face_state order
+ * (rog ug vg wg tg phig rop up vp wp tp pg pp phip), then the face centroid
+ * (0.3333 x node sum).  Reconstructed, not yet clipped or blended. */
+_Static_assert(sizeof(face_state) == 14 * sizeof(double), "face_state is 14 doubles");
+
+void hyo_face_states(hyo_case *c, double *out)
+{
+    for (int i = 0; i < c->neles; i++)
+        for (int f = 0; f < 4; f++) {
+            double xf[3];
+            face_state LR[2];
+            face_states(c, i, f, xf, &LR[0], &LR[1]);
+            for (int side = 0; side < 2; side++) {
+                double *o = out + ((size_t)(i * 4 + f) * 2 + side) * HF_N;
+                memcpy(o, &LR[side], sizeof(face_state));
+                o[14] = xf[0]; o[15] = xf[1]; o[16] = xf[2];
+            }
+        }
 }
 
 /* mod_Fluxmodule.F90:7-21 -> hllcbatten/hllczien: primitiveplusbc,

@@ -89,6 +89,10 @@ void hyo_bc(hyo_case *c);               /* sr_primitiveplusbc.F90:198-632 */
 void hyo_derivative(hyo_case *c);       /* sr_derivative.F90 */
 void hyo_venkat(hyo_case *c);           /* sr_VK.F90 */
 void hyo_face_flux(hyo_case *c);        /* face loop of hllcbatten/hllczien */
+/* reconstructed face states of every (cell, face, side) before clipping:
+ * out holds neles*4*2*HF_N doubles (14 fields, then the face centroid) */
+enum { HF_N = 17 };
+void hyo_face_states(hyo_case *c, double *out);
 void hyo_source(hyo_case *c);           /* sr_source.F90 */
 void hyo_flux_solver(hyo_case *c);      /* mod_Fluxmodule.F90:7-21 */
 void hyo_rk_stage(hyo_case *c, int s);  /* sr_RK.F90 stage s (1..4) update */

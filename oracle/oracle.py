@@ -41,6 +41,8 @@ def make_params(d: dict) -> Params:
 Q_NAMES = ["ug", "vg", "wg", "pg", "pp", "tg", "phig", "up", "vp", "wp", "phip", "tp", "rop", "rog",
            "egt", "ept", "ag", "ap"]
 G_NAMES = ["ug", "vg", "wg", "tg", "up", "vp", "wp", "tp", "pg", "pp", "rog", "rop", "phig"]
+# face_state fields (hyo_face_states)
+F_NAMES = ["rog", "ug", "vg", "wg", "tg", "phig", "rop", "up", "vp", "wp", "tp", "pg", "pp", "phip"]
 HA = {n: i for i, n in enumerate(["cn", "cu", "cn1", "rhs", "d24", "d2q", "anu", "q", "grad", "phi",
                                    "vis", "sc", "xc", "yc", "zc", "vol", "dl", "dd", "epssq"])}
 
@@ -80,6 +82,7 @@ def lib():
         L.hyo_set_donors.argtypes = [vp, ctypes.c_int, ip, ip, dp]
         L.hyo_donor_gather.argtypes = [vp]
         L.hyo_get_dt.restype = ctypes.c_double
+        L.hyo_face_states.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -164,6 +167,13 @@ class Oracle:
 
     def run(self, n):
         lib().hyo_run(self.h, int(n))
+
+    def face_states(self):
+        """Reconstructed left/right face states before clipping, shape
+        (neles, 4, 2, 17): face_state fields, then the face centroid."""
+        out = np.empty((self.ne, 4, 2, 17))
+        lib().hyo_face_states(self.h, _dp(out))
+        return out
 
     def set_donors(self, recv, donors, weights):
         """Overset donor lists, 0-based (see hyo_set_donors)."""

@@ -7,16 +7,28 @@
 ! The input layout is written by tests/golden/make_golden.py; kernel ids:
 !   1 primitive  2 primitiveplusbc  3 smooth  4 viscousgp
 !   5 venkatakrishnan  6 source  7 hllcbatten  8 hllczien
-! (7 and 8 with myorder=1 only: the second-order branch calls derivative,
+!   9 hdr(8) whole time steps composed from the reference's own routines
+! (7, 8 and 9 with myorder=1 only: the second-order branch calls derivative,
 ! which ends in the MPI halo exchange that is not part of this build.)
+!
+! Kernel 9 follows the step loop of main.F90:515-625: primitive, the CFL
+! loop of main.F90:520-551 and dt = cfl*dtmin (restated below), SMOOTH,
+! then RK (sr_RK.F90:19-164): VISCOUSGP once and four times the
+! reference's CallFluxSolver (mod_Fluxmodule.F90:7-21) followed by the
+! stage update (restated below from sr_RK.F90:27-54, 64-90, 97-125,
+! 134-161), then PRIMITIVE, the max|cn-cu| monitor (main.F90:603-612) and
+! cu = cn over ntot (main.F90:618-623).  The halo exchanges mpiexcn skip
+! every loop when nprocs = 1 and pghosts = 0 (sr_mpiexchangecn.F90:31-94),
+! so they are left out; MPI_Allreduce of one rank is the identity.
 program ref_harness
   use GlobalVariables
+  use FluxModule, only: CallFluxSolver
   implicit none
   external primitive, primitiveplusbc, smooth, viscousgp, venkatakrishnan, source
   external hllcbatten, hllczien
   character(len=512) :: fin, fout
-  integer :: hdr(8), kernel, i, u
-  real(dp) :: prm(44)
+  integer :: hdr(8), kernel, i, j, u, istep
+  real(dp) :: prm(44), qgi, qpi, dtbyvol
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -116,6 +128,73 @@ program ref_harness
   case (8)
     if (myorder /= 1) stop 'hllczien: harness runs myorder=1 only'
     call hllczien
+  case (9)
+    if (myorder /= 1) stop 'composed steps: harness runs myorder=1 only'
+    do istep = 1, hdr(8)
+      call primitive
+      dtmin = 1.0E8
+      total = total + 1
+      do i = 1, neles
+        ag(i) = SQRT((gamag * pg(i)) / rog(i))
+        ap(i) = SQRT((gamap * (pp(i) + pinf)) / rop(i))
+        qgi = SQRT(ug(i)**2 + vg(i)**2 + wg(i)**2)
+        qpi = SQRT(up(i)**2 + vp(i)**2 + wp(i)**2)
+        dtmin = MIN(dtmin, dl(i) / (MAX(ag(i), ap(i)) + MAX(qgi, qpi)))
+      end do
+      dt = dtmin * cfl
+      time = time + dt
+      call smooth
+      call viscousgp
+      call CallFluxSolver()
+      do i = 1, neles
+        if (iblank(i) /= 1) cycle
+        dtbyvol = dt / vol(i)
+        do j = 1, neq
+          cn(i, j) = cu(i, j) - (11.0D0 / 20.0D0) * dtbyvol * (rhs(i, j) - d24(i, j))
+          cn1(i, j) = cn(i, j)
+        end do
+      end do
+      call CallFluxSolver()
+      do i = 1, neles
+        if (iblank(i) /= 1) cycle
+        dtbyvol = dt / vol(i)
+        do j = 1, neq
+          cn(i, j) = (3.0D0/8.0D0) * cu(i, j) + (5.0D0/8.0D0) * (cn(i, j) - (11.0D0/20.0D0) * dtbyvol &
+                     * (rhs(i, j) - d24(i, j)))
+        end do
+      end do
+      call CallFluxSolver()
+      do i = 1, neles
+        if (iblank(i) /= 1) cycle
+        dtbyvol = dt / vol(i)
+        do j = 1, neq
+          cn(i, j) = (4.0D0/9.0D0) * cu(i, j) + (5.0D0/9.0D0) * (cn(i, j) - (11.0D0/20.0D0) * dtbyvol &
+                     * (rhs(i, j) - d24(i, j)))
+        end do
+      end do
+      call CallFluxSolver()
+      do i = 1, neles
+        if (iblank(i) /= 1) cycle
+        dtbyvol = dt / vol(i)
+        do j = 1, neq
+          cn(i, j) = (111.0D0/1331.0D0) * cu(i, j) + (260.0D0/1331.0D0) * cn1(i, j) + (960.0D0/1331.0D0) &
+                     * (cn(i, j) - (11.0D0 / 20.0D0) * dtbyvol * (rhs(i, j) - d24(i, j)))
+        end do
+      end do
+      call primitive
+      diffp = 0.0d0; diffu = 0.0d0; diffv = 0.0d0; diffw = 0.0d0
+      do i = 1, neles
+        diffp = max(diffp, abs(cn(i, 1) - cu(i, 1)))
+        diffu = max(diffu, abs(cn(i, 2) - cu(i, 2)))
+        diffv = max(diffv, abs(cn(i, 3) - cu(i, 3)))
+        diffw = max(diffw, abs(cn(i, 4) - cu(i, 4)))
+      end do
+      do i = 1, ntot
+        do j = 1, neq
+          cu(i, j) = cn(i, j)
+        end do
+      end do
+    end do
   case default
     stop 'unknown kernel id'
   end select
@@ -130,5 +209,6 @@ program ref_harness
            delphigx, delphigy, delphigz
   write(u) phi
   write(u) visug, visvg, viswg, vistg, visup, visvp, viswp, vistp
+  write(u) dt, time, real(total, dp), diffu, diffv, diffw, diffp
   close(u)
 end program ref_harness

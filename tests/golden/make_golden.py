@@ -14,6 +14,14 @@ from and what the reference routine leaves behind:
   second-order branch calls `derivative`, which ends in the MPI halo
   exchange and cannot run in this build).
 
+It also records whole first-order time steps composed from the reference's
+own routines (harness kernel 9: primitive, CFL loop, smooth, viscousgp,
+four CallFluxSolver + stage updates, primitive, monitor, cu = cn) from a
+restart state with total = 0, for both flux methods, into ref_steps.npz:
+the state after each of the first K steps, so the composition (BC history
+with the outflow start, frozen d24/vis, the double BC application of
+stage 1, the step order) is pinned step by step.
+
 The input state comes from the C oracle (a few steps from a water-sphere
 restart plus a seeded velocity perturbation), so every vector is
 reference-in / reference-out; the oracle is then checked against them by
@@ -40,7 +48,8 @@ import oracle as O  # noqa: E402
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 OUT = os.path.join(ROOT, "tests", "golden")
 KERNELS = {"primitive": 1, "primitiveplusbc": 2, "smooth": 3, "viscousgp": 4,
-           "venkatakrishnan": 5, "source": 6, "hllcbatten": 7, "hllczien": 8}
+           "venkatakrishnan": 5, "source": 6, "hllcbatten": 7, "hllczien": 8, "steps": 9}
+STEPS_K = 5
 PRM_ORDER = O.PARAM_D + O.PARAM_X
 
 # which outputs each routine writes (names of the harness output blocks)
@@ -107,10 +116,10 @@ def load_snapshot(o, s):
     o.set_dt(float(s["dt"]), float(s["time"]), int(s["total"]))
 
 
-def write_input(path, o, s, p, kernel, myorder):
+def write_input(path, o, s, p, kernel, myorder, nsteps=0):
     m = o.mesh
     ne, ng, nt = m.neles, m.nghosts, m.ntot
-    hdr = np.array([m.nodes, ne, ng, p["myflux"], myorder, int(s["total"]), kernel, 0], dtype=np.int32)
+    hdr = np.array([m.nodes, ne, ng, p["myflux"], myorder, int(s["total"]), kernel, nsteps], dtype=np.int32)
     d = M.derive(p)
     prm = np.array([d[n] for n in PRM_ORDER] + [float(s["dt"]), float(s["time"])], dtype=np.float64)
     assert prm.size == 44
@@ -163,15 +172,69 @@ def read_output(path, m):
     take("cn", (10, nt)); take("cu", (10, nt)); take("cn1", (10, nt)); take("d2q", (10, nt))
     take("rhs", (10, ne)); take("d24", (10, ne)); take("anu", (3, nt))
     take("q", (18, nt)); take("grad", (39, nt)); take("phi", (nt,)); take("vis", (8, ne))
+    take("trailer", (7,))
     assert off == raw.size, (off, raw.size)
+    t = out.pop("trailer")
+    out["dt"], out["time"], out["total"] = t[0], t[1], np.int64(t[2])
+    out["diffs"] = t[3:7].copy()  # diffu, diffv, diffw, diffp
     out["anu"] = out["anu"][:2]
     return out
+
+
+def run_harness(o, s, p, kernel, myorder, nsteps=0):
+    with tempfile.TemporaryDirectory() as td:
+        fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        write_input(fi, o, s, p, kernel, myorder, nsteps)
+        subprocess.run([HARNESS, fi, fo], check=True)
+        return read_output(fo, o.mesh)
+
+
+def restart_state(m, p, seed):
+    """Sphere restart plus a seeded momentum perturbation, loaded as
+    main.F90:427-482 does (cn = cu, ghost prims from the file, total = 0)."""
+    cu, gp = M.sphere_state(m, p, [(0.0, 0.1, 0.0, 0.22)])
+    rng = np.random.default_rng(seed)
+    for k in (1, 2, 3):
+        cu[k] += cu[0] * rng.uniform(-30.0, 30.0, m.neles)
+        cu[5 + k] += cu[5] * rng.uniform(-30.0, 30.0, m.neles)
+    gp[1] = rng.uniform(-20.0, 20.0, m.nghosts)
+    gp[2] = rng.uniform(-20.0, 20.0, m.nghosts)
+    return cu, gp
+
+
+def composed_steps(m):
+    """Reference-composed first-order steps 1..STEPS_K, both fluxes."""
+    fixtures = {}
+    for flux in (2, 1):
+        p = golden_params(myflux=flux, myorder=1)
+        cu, gp = restart_state(m, p, seed=21)
+        o = O.Oracle(m, p)
+        o.set_state(cu, gp, 0.0, 0)
+        s = snapshot(o)
+        s["dt"] = np.float64(0.0)
+        fixtures[f"f{flux}/in/cu"] = cu
+        fixtures[f"f{flux}/in/gp"] = gp
+        for k in range(1, STEPS_K + 1):
+            ref = run_harness(o, s, p, KERNELS["steps"], 1, nsteps=k)
+            ne = m.neles
+            fixtures[f"f{flux}/step{k}/cu"] = ref["cu"][:, :ne].copy()
+            fixtures[f"f{flux}/step{k}/gp"] = ref["q"][[0, 1, 2, 3, 5, 13], ne:].copy()
+            fixtures[f"f{flux}/step{k}/q"] = ref["q"][:16].copy()
+            fixtures[f"f{flux}/step{k}/dt"] = np.float64(ref["dt"])
+            fixtures[f"f{flux}/step{k}/time"] = np.float64(ref["time"])
+            fixtures[f"f{flux}/step{k}/total"] = np.int64(ref["total"])
+            fixtures[f"f{flux}/step{k}/diffs"] = ref["diffs"]
+    return fixtures
 
 
 def main():
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     m = golden_mesh()
+    composed = composed_steps(m)
+    path = os.path.join(OUT, "ref_steps.npz")
+    np.savez_compressed(path, **composed)
+    print("wrote", path, os.path.getsize(path), "bytes")
     fixtures = {}
     for scen, (seed, steps) in {"s1": (11, 0), "s3": (12, 2)}.items():
         for flux in (2, 1):
@@ -180,17 +243,13 @@ def main():
             if flux == 1:  # the state is the same; only the Zien kernel is new
                 kern = ["hllczien"]
             else:
-                kern = [k for k in KERNELS if k != "hllczien"]
+                kern = [k for k in KERNELS if k not in ("hllczien", "steps")]
             # gradients + limiter present in the state for VK/source inputs
             o.call("derivative")
             base = snapshot(o)
             for k in kern:
                 myorder = 1 if k in ("hllcbatten", "hllczien") else 2
-                with tempfile.TemporaryDirectory() as td:
-                    fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
-                    write_input(fi, o, base, p, KERNELS[k], myorder)
-                    subprocess.run([HARNESS, fi, fo], check=True)
-                    ref = read_output(fo, m)
+                ref = run_harness(o, base, p, KERNELS[k], myorder)
                 for w in WRITES[k]:
                     fixtures[f"{scen}/{k}/out/{w}"] = ref[w]
             for n, v in base.items():

@@ -65,3 +65,28 @@ def test_oracle_matches_reference_routine(scen, kernel):
             # interior cells and ghost cells the routine writes
             pass
         np.testing.assert_array_equal(got, ref, err_msg=f"{scen}/{kernel}: {name}")
+
+
+STEPS = np.load(os.path.join(HERE, "golden", "ref_steps.npz"))
+
+
+@pytest.mark.parametrize("flux", [2, 1])
+def test_oracle_steps_match_reference_composed_steps(flux):
+    """Whole first-order time steps: the oracle's hyo_step against the
+    reference's own routines composed in main.F90:515-625 order (harness
+    kernel 9), from a restart with total = 0, after each of five steps:
+    conserved state, ghost history, primitives, dt, time, step counter and
+    the max|cn-cu| monitor, bit for bit."""
+    m = golden_mesh()
+    o = O.Oracle(m, params(flux, 1))
+    o.set_state(STEPS[f"f{flux}/in/cu"], STEPS[f"f{flux}/in/gp"], 0.0, 0)
+    for k in range(1, 6):
+        pre = f"f{flux}/step{k}/"
+        d = o.step()
+        cu, gp, t, tot = o.get_state()
+        np.testing.assert_array_equal(cu, STEPS[pre + "cu"], err_msg=f"step {k} cu")
+        np.testing.assert_array_equal(gp, STEPS[pre + "gp"], err_msg=f"step {k} ghosts")
+        np.testing.assert_array_equal(o.array("q")[:16], STEPS[pre + "q"], err_msg=f"step {k} q")
+        assert o.dt == float(STEPS[pre + "dt"]) and t == float(STEPS[pre + "time"])
+        assert tot == int(STEPS[pre + "total"]) == k
+        np.testing.assert_array_equal(d, STEPS[pre + "diffs"], err_msg=f"step {k} monitor")
