@@ -129,6 +129,7 @@ const char *hp_last_error(const hp_ctx *ctx);
  *        7 primitives(14: ug vg wg pg pp tg phig up vp wp phip tp rop rog;
  *          ntot rows) 8 gradients(33: ug vg wg tg up vp wp tp pg pp phig,
  *          each x y z) 9 phi 10 vis(8) 11 dt/time/total (3 doubles)
+ *        12 dl (minimum edge length per cell; read-only)
  * The array is variable-major with leading dimension neles (ntot for 7). */
 int hp_get_array(hp_ctx *ctx, int which, double *host, size_t count);
 int hp_set_array(hp_ctx *ctx, int which, const double *host, size_t count);
@@ -197,6 +198,28 @@ size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision);
  * shared-divisor quotient differs in any bit from the hardware a[k] / b[k]
  * (must be 0).  Allocates and synchronises; errors in hp_last_error(NULL). */
 int hp_check_division(const double *a, const double *b, size_t n, size_t *mismatches);
+
+/* ---------------------------------------------------------------------------
+ * No-allocation entry point.  No counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops that forbid allocation): hp_dt_min's CFL min-reduction
+ * (main.F90:520-548) over device arrays the caller owns, enqueued on
+ * `stream` (NULL: the null stream), with the result left in device memory
+ * and no host synchronisation.
+ *   q(ldq, 14): device primitives in the order of hp_get_array id 7
+ *   dl(neles): device minimum edge lengths (hp_get_array id 12)
+ *   dtmin: one device double receiving min over cells of
+ *          dl / (max(ag, ap) + max(|u_g|, |u_p|))
+ * The call allocates nothing: its scratch (one partial minimum per 256
+ * cells) is a fixed 2 MiB device array of the library that each GPU holds
+ * from set-up (hp_create, or the first call on the device) until the
+ * process ends, so neles may not exceed HP_NOALLOC_MAX_CELLS (HP_ERR_ARG).
+ * Calls on one GPU share that array: calls that overlap in time on one GPU
+ * (different streams, or graphs replayed concurrently) race; calls ordered
+ * on one stream do not.  The stream must belong to the current device. */
+#define HP_NOALLOC_MAX_CELLS 67108864
+int hp_dt_min_noalloc(const double *q, size_t ldq, const double *dl, int neles, double gamag,
+                      double gamap, double pinf, double *dtmin, void *stream);
 
 #ifdef __cplusplus
 }

@@ -100,6 +100,12 @@ int fail(hp_ctx *c, int code, const std::string &msg)
     return code;
 }
 
+}  // namespace
+
+int hydra::set_global_error(int code, const std::string &msg) { return fail(nullptr, code, msg); }
+
+namespace {
+
 #define HIPCHK(c, call)                                                                           \
     do {                                                                                          \
         hipError_t e_ = (call);                                                                   \
@@ -302,6 +308,11 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
         zmin = std::min(zmin, z[n]);
     }
 
+    {
+        // set-up of the no-allocation entry points on this device
+        std::string err;
+        if (int rc0 = hydra::noalloc_setup(&err)) return fail(nullptr, rc0, err);
+    }
     hp_ctx *c = new hp_ctx();
     c->P = *params;
     if (c->P.rrg == 0.0) derive_params(c->P);
@@ -678,6 +689,7 @@ static int array_view(hp_ctx *c, int which, double **ptr, size_t *n)
     case 8: *ptr = c->s.grad; *n = (size_t)3 * G_N * ne; return HP_OK;
     case 9: *ptr = c->s.phi; *n = ne; return HP_OK;
     case 10: *ptr = c->s.vis; *n = 8 * ne; return HP_OK;
+    case 12: *ptr = c->m.dl; *n = ne; return HP_OK;
     default: return fail(c, HP_ERR_ARG, "unknown array id");
     }
 }
@@ -707,6 +719,7 @@ int hp_get_array(hp_ctx *c, int which, double *host, size_t count)
 int hp_set_array(hp_ctx *c, int which, const double *host, size_t count)
 {
     if (!c || !host) return fail(c, HP_ERR_ARG, "hp_set_array: bad argument");
+    if (which == 12) return fail(c, HP_ERR_ARG, "hp_set_array: array 12 (dl) is read-only");
     double *p;
     size_t n;
     if (int rc = array_view(c, which, &p, &n)) return rc;

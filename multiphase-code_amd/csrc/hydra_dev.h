@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "hydra_hip.h"
 
 namespace hydra {
@@ -21,6 +23,23 @@ enum { SRC_MX, SRC_MY, SRC_MZ, SRC_E, SRC_MXP, SRC_N };
 enum { G_UG, G_VG, G_WG, G_TG, G_UP, G_VP, G_WP, G_TP, G_PG, G_PP, G_PHIG, G_N };
 // device scalars
 enum { SC_DT, SC_TIME, SC_DTMIN, SC_N };
+
+// CFL bound of one cell from its primitives (main.F90:525-546): dl over the
+// larger phase sound speed plus the larger phase speed.  q is variable-major
+// with leading dimension nt.
+__device__ __forceinline__ double cfl_bound(const double *q, size_t nt, const double *dl, int i,
+                                            double gamag, double gamap, double pinf)
+{
+    const double ag = sqrt((gamag * q[(size_t)Q_PG * nt + i]) / q[(size_t)Q_ROG * nt + i]);
+    const double ap = sqrt((gamap * (q[(size_t)Q_PP * nt + i] + pinf)) / q[(size_t)Q_ROP * nt + i]);
+    const double ug = q[(size_t)Q_UG * nt + i], vg = q[(size_t)Q_VG * nt + i], wg = q[(size_t)Q_WG * nt + i];
+    const double up = q[(size_t)Q_UP * nt + i], vp = q[(size_t)Q_VP * nt + i], wp = q[(size_t)Q_WP * nt + i];
+    const double qgi = sqrt(ug * ug + vg * vg + wg * wg);
+    const double qpi = sqrt(up * up + vp * vp + wp * wp);
+    const double a = ag > ap ? ag : ap;
+    const double v = qgi > qpi ? qgi : qpi;
+    return dl[i] / (a + v);
+}
 
 // mesh and geometry, all device pointers, 0-based ids
 struct Mesh {
@@ -130,6 +149,12 @@ struct Donors {
 // eos: 0 the gather from cn alone; fused step, 1 the gather from cnd, then
 // the receptors' primitive recovery into s.q, 2 as 1 plus cu = cn (last stage)
 void launch_donor_gather(const Params &P, const State &s, const Donors &D, int eos, hipStream_t st);
+
+// the no-allocation entry points' scratch array on the current device:
+// resolves its address once per device (hydra_noalloc.hip); 0 or an hp_status
+int noalloc_setup(std::string *err);
+// the error hp_last_error(NULL) reports; returns code
+int set_global_error(int code, const std::string &msg);
 
 }  // namespace hydra
 

@@ -1372,15 +1372,7 @@ namespace hydra {
 // CFL bound of one cell from the current primitives (main.F90:525-546)
 __device__ __forceinline__ double dt_cell(const Params &P, const State &s, int i)
 {
-    const double *q = s.q;
-    const size_t nt = s.nt;
-    const double ag = sqrt((P.gamag * QV(Q_PG, i)) / QV(Q_ROG, i));
-    const double ap = sqrt((P.gamap * (QV(Q_PP, i) + P.pinf)) / QV(Q_ROP, i));
-    const double ug = QV(Q_UG, i), vg = QV(Q_VG, i), wg = QV(Q_WG, i);
-    const double up = QV(Q_UP, i), vp = QV(Q_VP, i), wp = QV(Q_WP, i);
-    const double qgi = sqrt(ug * ug + vg * vg + wg * wg);
-    const double qpi = sqrt(up * up + vp * vp + wp * wp);
-    return s.dl[i] / (dmax(ag, ap) + dmax(qgi, qpi));
+    return cfl_bound(s.q, (size_t)s.nt, s.dl, i, P.gamag, P.gamap, P.pinf);
 }
 
 }  // namespace hydra

@@ -17,6 +17,8 @@ PARAM_D = ["pmax", "pmin", "tgmax", "tgmin", "ugmax", "ugmin", "vgmax", "vgmin",
            "epslnmax", "epslnmin", "epslninf", "phipzero",
            "cfl", "diap", "surfacetension", "as_", "ts"]
 PARAM_I = ["niter", "itersub", "istart", "myflux", "myorder", "precision"]
+# largest neles hp_dt_min_noalloc takes (include/hydra_hip.h)
+NOALLOC_MAX_CELLS = 67108864
 # hp_health status bits (include/hydra_hip.h)
 HEALTH_NONFINITE = 1
 HEALTH_PRESSURE = 2
@@ -47,7 +49,7 @@ EXPORTS = ["hp_create", "hp_destroy", "hp_upload_state", "hp_download_state", "h
            "hp_set_array", "hp_last_timing", "hp_probe_cell", "hp_health", "hp_batch_health", "hp_check_division",
            "hp_batch_create", "hp_batch_destroy", "hp_batch_add", "hp_batch_remove", "hp_batch_set_grid",
            "hp_batch_step", "hp_batch_synchronize", "hp_batch_last_ms", "hp_batch_last_error",
-           "hp_case_bytes", "hp_set_donors", "hp_apply_donors"]
+           "hp_case_bytes", "hp_set_donors", "hp_apply_donors", "hp_dt_min_noalloc"]
 
 
 def lib():
@@ -93,6 +95,7 @@ def lib():
     L.hp_batch_last_error.restype = ctypes.c_char_p
     L.hp_set_donors.argtypes = [vp, c_int, ip, ip, c_int, dp]
     L.hp_apply_donors.argtypes = [vp]
+    L.hp_dt_min_noalloc.argtypes = [vp, c_size, vp, c_int, c_double, c_double, c_double, vp, vp]
     L.hp_case_bytes.argtypes = [c_int, c_int, c_int, c_int]
     L.hp_case_bytes.restype = c_size
     for n in EXPORTS:

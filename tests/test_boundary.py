@@ -101,3 +101,20 @@ def test_health_and_division_check_arguments():
     assert L.hp_check_division(None, None, 0, ctypes.byref(n)) == 0 and n.value == 0
     assert L.hp_check_division(None, None, 4, ctypes.byref(n)) == 1
     assert "hp_check_division" in L.hp_last_error(None).decode()
+
+
+def test_dt_min_noalloc_rejects_bad_arguments_before_any_device_call():
+    """hp_dt_min_noalloc checks its arguments on the host first: null
+    arrays, ldq < neles, neles <= 0 and neles above HP_NOALLOC_MAX_CELLS are
+    HP_ERR_ARG with a message (no device is touched, so this runs without a
+    GPU)."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: rejected before use
+    mx = _lib.NOALLOC_MAX_CELLS
+    cases = [(None, 10, fake, 10, fake), (fake, 10, None, 10, fake), (fake, 10, fake, 10, None),
+             (fake, 5, fake, 10, fake), (fake, 10, fake, 0, fake), (fake, mx + 1, fake, mx + 1, fake)]
+    for q, ldq, dl, n, out in cases:
+        rc = L.hp_dt_min_noalloc(q, ldq, dl, n, 1.4, 4.4, 6e8, out, None)
+        assert rc == 1, (ldq, n, rc)
+        assert L.hp_last_error(None)
+    assert b"HP_NOALLOC_MAX_CELLS" in L.hp_last_error(None)
