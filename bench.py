@@ -22,15 +22,18 @@ primitiveplusbc + gradients + limiter + HLLC face flux + source + update
 stages) per second with the state resident in HBM.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
-       python bench.py summarize-counters FETCH.csv WRITE.csv CALIB_MB
+       python bench.py summarize-counters READREQ.csv WRITEREQ.csv CALIB_MB
        python bench.py summarize-trace results.db [KERNEL-FOR-A-PER-GRID-SPLIT]
        python bench.py summarize-instructions counter_collection.csv
+       python bench.py summarize-pmc counter_collection.csv
 
-roofline.traffic is the HBM bytes of one k_flux launch measured with
-rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (each run with
---calib-copy-mb 1024 and summarised by summarize-counters into
-profiles/k_flux_overset96x96x90_fp64_counters.md); FETCH_SIZE and WRITE_SIZE
-are scaled by what they read for a float64 b=2a kernel of known size.
+roofline.traffic is the HBM bytes of one k_flux launch measured with two
+rocprofv3 --pmc runs, of the L2's read requests by size
+(TCC_EA0_RDREQ_32B/_64B/_128B, the terms of FETCH_SIZE) and of its write
+requests (TCC_EA0_WRREQ/_64B, those of WRITE_SIZE), each run with
+--calib-copy-mb 1024 (the library's streaming copy kernel, of known traffic)
+and summarised by summarize-counters into
+profiles/k_flux_overset96x96x90_fp64_counters.md.
 """
 from __future__ import annotations
 
@@ -58,6 +61,17 @@ FLUX_BYTES_PER_CELL = 112 + 264 + 8 + 96 + 96 + 24 + 16 + 8 + 40 + 64 + 80 + 80 
 # FP32-bulk form: gradients 33x4 and limiter 4 instead of 33x8 and 8; the
 # source is formed in the flux from vol+dl (16) instead of read (8 + 40)
 FLUX_BYTES_PER_CELL_F32 = FLUX_BYTES_PER_CELL - 132 - 4 + 16 - 48
+# The same launch priced by the path's own counting rule, independent of
+# this library's tables: the face-flux pass reads primitives 104, gradients
+# 264, limiter 8, area vectors 96, centroid 24, node ids 16, node
+# coordinates 4 (a node per six cells), neighbour ids 16, vol 8, dl 8,
+# viscous flux 64, d24 80, cu 80, cn 80 and cn1 20 (stage 4, a quarter),
+# writes cn 80 and cn1 20 (stage 1, a quarter): 972 B; the primitive
+# recovery fused into it reads cn 80 and writes 13 primitives 104: 184 B.
+# roofline.achieved and .frac use these 1,156 B per active cell.
+FLUX_BYTES_RULE = 972 + 184
+# FP32 bulk under the same rule: gradients 132 and limiter 4
+FLUX_BYTES_RULE_F32 = FLUX_BYTES_RULE - 132 - 4
 
 
 def build_case(nx, ny, nz, seed=1, precision=0, overset=True, slabs=(8, 8)):
@@ -86,49 +100,17 @@ def build_case(nx, ny, nz, seed=1, precision=0, overset=True, slabs=(8, 8)):
 
 
 def stream_copy_gbs(mib=1024, reps=10):
-    """Measured copy ceiling: a device-to-device copy of mib MiB (reads and
-    writes mib MiB each), timed with HIP events over reps copies, through the
-    HIP runtime the library already loaded.  GB/s of read + write bytes."""
+    """Measured copy ceiling: the library's hand-written streaming copy kernel
+    (hp_stream_bandwidth) copies mib MiB reps times; GB/s of read + write
+    bytes."""
     import ctypes
-    hip = ctypes.CDLL("libamdhip64.so.7")
-    vp, sz = ctypes.c_void_p, ctypes.c_size_t
-    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
-    hip.hipFree.argtypes = [vp]
-    hip.hipMemset.argtypes = [vp, ctypes.c_int, sz]
-    hip.hipMemcpyAsync.argtypes = [vp, vp, sz, ctypes.c_int, vp]
-    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
-    hip.hipEventRecord.argtypes = [vp, vp]
-    hip.hipEventSynchronize.argtypes = [vp]
-    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
-    hip.hipEventDestroy.argtypes = [vp]
-
-    def chk(rc, what):
-        if rc != 0:
-            raise RuntimeError(f"{what} failed ({rc})")
-    n = mib << 20
-    a, b, e0, e1 = vp(), vp(), vp(), vp()
-    chk(hip.hipMalloc(ctypes.byref(a), n), "hipMalloc")
-    try:
-        chk(hip.hipMalloc(ctypes.byref(b), n), "hipMalloc")
-        try:
-            chk(hip.hipMemset(a, 1, n), "hipMemset")
-            chk(hip.hipEventCreate(ctypes.byref(e0)), "hipEventCreate")
-            chk(hip.hipEventCreate(ctypes.byref(e1)), "hipEventCreate")
-            chk(hip.hipMemcpyAsync(b, a, n, 3, None), "hipMemcpyAsync")  # warm-up
-            chk(hip.hipEventRecord(e0, None), "hipEventRecord")
-            for _ in range(reps):
-                chk(hip.hipMemcpyAsync(b, a, n, 3, None), "hipMemcpyAsync")
-            chk(hip.hipEventRecord(e1, None), "hipEventRecord")
-            chk(hip.hipEventSynchronize(e1), "hipEventSynchronize")
-            ms = ctypes.c_float()
-            chk(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1), "hipEventElapsedTime")
-            hip.hipEventDestroy(e0)
-            hip.hipEventDestroy(e1)
-        finally:
-            hip.hipFree(b)
-    finally:
-        hip.hipFree(a)
-    return 2.0 * n * reps / (ms.value * 1e-3) / 1e9
+    from hydraphase import _lib
+    L = _lib.lib()
+    gbs, ms = ctypes.c_double(), ctypes.c_double()
+    rc = L.hp_stream_bandwidth(mib << 20, reps, ctypes.byref(gbs), ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"hp_stream_bandwidth failed: {L.hp_last_error(None).decode()}")
+    return gbs.value
 
 
 def cpu_baseline(m, p, cu, gp, donors=None, seconds_target=20.0):
@@ -173,45 +155,82 @@ def flux_traffic_bytes():
     return int(m.group(1)) if m else None
 
 
-def summarize_counters(fetch_csv, write_csv, calib_mb):
-    """Per-kernel FETCH_SIZE/WRITE_SIZE means (KiB per dispatch) from two
-    rocprofv3 --pmc runs, scaled by the same counters on the b=2a calibration
-    kernel that reads and writes calib_mb MiB each.  Prints markdown."""
+def _pmc_means(path):
+    """{kernel: {counter: mean per dispatch}} and {kernel: dispatches} of a
+    rocprofv3 --pmc counter_collection.csv."""
     import csv
     import collections
+    tot = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(set)
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            tot[r["Kernel_Name"]][r["Counter_Name"]] += float(r["Counter_Value"])
+            disp[r["Kernel_Name"]].add(r["Dispatch_Id"])
+    return ({k: {c: v / len(disp[k]) for c, v in cs.items()} for k, cs in tot.items()},
+            {k: len(v) for k, v in disp.items()})
 
-    def load(path):
-        agg = collections.defaultdict(list)
-        with open(path) as fh:
-            for r in csv.DictReader(fh):
-                agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
-        return {k: sum(v) / len(v) for k, v in agg.items()}, {k: len(v) for k, v in agg.items()}
 
-    fe, nfe = load(fetch_csv)
-    wr, _ = load(write_csv)
-    cal = [k for k in fe if "elementwise" in k and ("Mul" in k or "Unary" in k)]
+def summarize_counters(read_csv, write_csv, calib_mb):
+    """HBM bytes per dispatch from the L2's memory requests by size: a run of
+    TCC_EA0_RDREQ_32B/_64B/_128B (and the total TCC_EA0_RDREQ) and a run of
+    TCC_EA0_WRREQ/_64B.  Read bytes = 32 n32 + 64 n64 + 128 n128, written =
+    32 (n - n64) + 64 n64.  The calibration kernel (hydra::k_stream_copy,
+    reads and writes calib_mb MiB) checks the accounting; its ratio of known
+    to counted bytes is printed and applied.  Prints markdown."""
+    rd, nrd = _pmc_means(read_csv)
+    wr, _ = _pmc_means(write_csv)
+
+    def rbytes(c):
+        return (32 * c.get("TCC_EA0_RDREQ_32B_sum", 0.0) + 64 * c.get("TCC_EA0_RDREQ_64B_sum", 0.0)
+                + 128 * c.get("TCC_EA0_RDREQ_128B_sum", 0.0))
+
+    def wbytes(c):
+        n, n64 = c.get("TCC_EA0_WRREQ_sum", 0.0), c.get("TCC_EA0_WRREQ_64B_sum", 0.0)
+        return 32 * (n - n64) + 64 * n64
+
+    cal = [k for k in rd if "k_stream_copy" in k]
     if not cal:
-        raise SystemExit("calibration kernel not found in " + fetch_csv)
-    kib = calib_mb * 1024.0
-    ffac = kib / fe[cal[0]]
-    wfac = kib / wr[cal[0]]
-    print(f"calibration kernel `{cal[0][:60]}...`: reads and writes {calib_mb} MiB; FETCH_SIZE "
-          f"{fe[cal[0]]:.0f} KiB, WRITE_SIZE {wr[cal[0]]:.0f} KiB -> factors {ffac:.3f} (fetch), "
-          f"{wfac:.3f} (write)\n")
-    print("| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM read MB (scaled) | HBM write MB (scaled) |")
-    print("|---|---|---|---|---|---|")
+        raise SystemExit("calibration kernel hydra::k_stream_copy not found in " + read_csv)
+    known = calib_mb * float(1 << 20)
+    rfac = known / rbytes(rd[cal[0]])
+    wfac = known / wbytes(wr[cal[0]])
+    c = rd[cal[0]]
+    print(f"calibration kernel `hydra::k_stream_copy`: reads and writes {calib_mb} MiB; counted "
+          f"{rbytes(c) / known:.4f} x read ({c.get('TCC_EA0_RDREQ_128B_sum', 0):.0f} x 128 B, "
+          f"{c.get('TCC_EA0_RDREQ_64B_sum', 0):.0f} x 64 B, {c.get('TCC_EA0_RDREQ_32B_sum', 0):.0f} x 32 B "
+          f"requests) and {wbytes(wr[cal[0]]) / known:.4f} x written -> factors {rfac:.4f} (read), "
+          f"{wfac:.4f} (write)\n")
+    print("| kernel | dispatches | read req 128/64/32 B (share of requests) | HBM read MB | HBM write MB |")
+    print("|---|---|---|---|---|")
     flux = None
-    for k in fe:
-        if "hydra" not in k:
+    for k in rd:
+        if "hydra" not in k or "k_stream_copy" in k:
             continue
-        rd = fe[k] * ffac * 1024 / 1e6
-        wt = wr.get(k, 0.0) * wfac * 1024 / 1e6
+        c = rd[k]
+        n = max(c.get("TCC_EA0_RDREQ_sum", 0.0), 1.0)
+        mix = "/".join(f"{c.get(f'TCC_EA0_RDREQ_{b}B_sum', 0.0) / n:.2f}" for b in (128, 64, 32))
+        r = rbytes(c) * rfac / 1e6
+        w = wbytes(wr.get(k, {})) * wfac / 1e6
         name = k.split("(")[0].replace("void ", "")
-        print(f"| `{name}` | {nfe[k]} | {fe[k]:.0f} | {wr.get(k, 0.0):.0f} | {rd:.1f} | {wt:.1f} |")
+        print(f"| `{name}` | {nrd[k]} | {mix} | {r:.1f} | {w:.1f} |")
         if name.startswith("hydra::k_flux"):
-            flux = int(round((rd + wt) * 1e6))
+            flux = int(round((r + w) * 1e6))
     if flux is not None:
         print(f"\ntraffic_bytes_per_launch k_flux: {flux}")
+
+
+def summarize_pmc(path, kernels=("k_flux", "k_grad", "k_viscous", "k_stream_copy")):
+    """Mean per dispatch of every counter of one rocprofv3 --pmc run, for the
+    kernels named (substrings).  Prints markdown."""
+    means, nd = _pmc_means(path)
+    names = sorted({c for cs in means.values() for c in cs})
+    print("| kernel | dispatches | " + " | ".join(names) + " |")
+    print("|---|---|" + "---|" * len(names))
+    for k, cs in means.items():
+        if not any(s in k for s in kernels):
+            continue
+        name = k.split("(")[0].replace("void ", "")
+        print(f"| `{name}` | {nd[k]} | " + " | ".join(f"{cs.get(c, 0.0):.4g}" for c in names) + " |")
 
 
 def summarize_instructions(sq_csv):
@@ -341,6 +360,9 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-counters":
         summarize_counters(sys.argv[2], sys.argv[3], int(sys.argv[4]))
         return
+    if len(sys.argv) > 1 and sys.argv[1] == "summarize-pmc":
+        summarize_pmc(sys.argv[2])
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-instructions":
         summarize_instructions(sys.argv[2])
         return
@@ -361,8 +383,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop run")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
-                    help="first run a plain device copy (b=2a, float64) of this many MiB, the "
-                         "known-size kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
+                    help="first run the streaming copy kernel over this many MiB, the kernel of "
+                         "known traffic that calibrates the HBM request counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every case runs "
@@ -370,16 +392,9 @@ def main():
         sys.exit(2)
 
     if args.calib_copy_mb:
-        # b = 2*a over float64: one elementwise kernel that reads and writes
-        # exactly calib_copy_mb MiB each, the yardstick for the counters
-        import torch
-        n = args.calib_copy_mb * (1 << 20) // 8
-        a = torch.ones(n, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            torch.mul(a, 2.0, out=b)
-        torch.cuda.synchronize()
-        del a, b
+        # hydra::k_stream_copy: one kernel of known traffic (reads and writes
+        # exactly calib_copy_mb MiB per launch), the yardstick for the counters
+        stream_copy_gbs(args.calib_copy_mb, 1)
     t0 = time.time()
     m, p, cu, gp, donors = build_case(*args.n, precision=args.precision, overset=not args.single_mesh)
     print(f"# case built: {m.neles} tets, {m.nghosts} ghosts, "
@@ -406,7 +421,8 @@ def main():
     ms_per_step = total_ms / args.steps
     value = updates / (total_ms * 1e-3)
     flux_avg_ms = flux_ms / max(nflux, 1)
-    bpc = FLUX_BYTES_PER_CELL if args.precision == 0 else FLUX_BYTES_PER_CELL_F32
+    bpc = FLUX_BYTES_RULE if args.precision == 0 else FLUX_BYTES_RULE_F32
+    layout_bpc = FLUX_BYTES_PER_CELL if args.precision == 0 else FLUX_BYTES_PER_CELL_F32
     achieved = bpc * active / (flux_avg_ms * 1e-3) / 1e9
     out = {
         "metric": "cell-stage updates/s @5M tets, %HBM peak; serve p50/p99 solve latency",
@@ -434,7 +450,9 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (flux_traffic_bytes() if list(args.n) == [96, 96, 90] and args.precision == 0
                                  and not args.single_mesh else None),
-                     "flux_avg_ms": flux_avg_ms, "bytes_per_cell": bpc},
+                     "flux_avg_ms": flux_avg_ms, "bytes_per_cell": bpc,
+                     "layout_bytes_per_cell": layout_bpc,
+                     "frac_on_layout_bytes": layout_bpc * active / (flux_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
     s.close()
     # measured copy ceiling beside the published peak

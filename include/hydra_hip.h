@@ -199,6 +199,16 @@ size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision);
  * (must be 0).  Allocates and synchronises; errors in hp_last_error(NULL). */
 int hp_check_division(const double *a, const double *b, size_t n, size_t *mismatches);
 
+/* Streaming copy rate of the current device (no counterpart in the
+ * reference; the ceiling the hot path's HBM rates are read against): a
+ * hand-written kernel copies `bytes` (a multiple of 16) from one device
+ * buffer to another `reps` times after one warm-up copy, timed with HIP
+ * events; *gbs = read + write bytes per second / 1e9, *ms_per_copy (may be
+ * NULL) the mean time of one copy.  Allocates both buffers and frees them;
+ * synchronises.  The kernel (hydra::k_stream_copy) reads and writes exactly
+ * `bytes` per launch, which calibrates the HBM byte counters. */
+int hp_stream_bandwidth(size_t bytes, int reps, double *gbs, double *ms_per_copy);
+
 /* ---------------------------------------------------------------------------
  * No-allocation entry point.  No counterpart in the reference; for callers
  * that cannot allocate device memory (code under HIP graph capture, serving

@@ -49,7 +49,8 @@ EXPORTS = ["hp_create", "hp_destroy", "hp_upload_state", "hp_download_state", "h
            "hp_set_array", "hp_last_timing", "hp_probe_cell", "hp_health", "hp_batch_health", "hp_check_division",
            "hp_batch_create", "hp_batch_destroy", "hp_batch_add", "hp_batch_remove", "hp_batch_set_grid",
            "hp_batch_step", "hp_batch_synchronize", "hp_batch_last_ms", "hp_batch_last_error",
-           "hp_case_bytes", "hp_set_donors", "hp_apply_donors", "hp_dt_min_noalloc"]
+           "hp_case_bytes", "hp_set_donors", "hp_apply_donors", "hp_dt_min_noalloc",
+           "hp_stream_bandwidth"]
 
 
 def lib():
@@ -96,6 +97,7 @@ def lib():
     L.hp_set_donors.argtypes = [vp, c_int, ip, ip, c_int, dp]
     L.hp_apply_donors.argtypes = [vp]
     L.hp_dt_min_noalloc.argtypes = [vp, c_size, vp, c_int, c_double, c_double, c_double, vp, vp]
+    L.hp_stream_bandwidth.argtypes = [c_size, c_int, dp, dp]
     L.hp_case_bytes.argtypes = [c_int, c_int, c_int, c_int]
     L.hp_case_bytes.restype = c_size
     for n in EXPORTS:

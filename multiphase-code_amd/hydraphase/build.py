@@ -15,7 +15,7 @@ REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libhydra_hip.so")
-SOURCES = ["hydra_kernels.hip", "hydra_capi.cpp", "hydra_noalloc.hip"]
+SOURCES = ["hydra_kernels.hip", "hydra_capi.cpp", "hydra_noalloc.hip", "hydra_stream.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
