@@ -305,27 +305,32 @@ def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
         upd += 4 * baseline_steps * int(m.iblank.sum())
         s.close()
     plain = upd / (ms * 1e-3)
-    errs = []
+    errs, floors = [], []
     for k in np.linspace(0, len(specs) - 1, sample).astype(int):
         m, p, cu, gp, _ = HB.build_case(specs[k], 0)
-        s = Solver(m, p)
-        s.upload(cu, gp, 0.0, 0)
-        s.step(steps)
-        ref = s.download()[0]
-        s.close()
+        fp64 = []
+        for start in (cu, SV.perturbed(cu, 2.0 ** -24)):
+            s = Solver(m, p)
+            s.upload(start, gp, 0.0, 0)
+            s.step(steps)
+            fp64.append(s.download()[0])
+            s.close()
+        ref = fp64[0]
         got = res["states"][k][0]
         if res["perms"][k] is not None:
             un = np.empty_like(got)
             un[:, res["perms"][k]] = got
             got = un
         errs.append(float(SV.mixture_rel_l2(got, ref).max()))
+        floors.append(float(SV.mixture_rel_l2(fp64[1], ref).max()))
     cells = sum(int(mm.iblank.sum()) for mm in res["meshes"])
     return {"workload": f"64 boxes n^3 x 6 tets, n log-uniform 8..48 ({cells} tets), Mach 0.1-0.8, "
                         f"R 0.05-0.25, half viscous, {steps} steps each",
             "updates_per_s": batched, "ms_per_step": res["ms"] / steps,
             "plain_fp64_one_case_updates_per_s": plain, "speedup_vs_plain_fp64": batched / plain,
             "tuning": {"morton": tuning["morton"], "slabs": tuning["slabs"], "grid": list(tuning["grid"])},
-            "mixture_rel_l2_vs_fp64_sample": errs, "sample_cases": sample,
+            "mixture_rel_l2_vs_fp64_sample": errs,
+            "mixture_rel_l2_floor_sample": floors, "sample_cases": sample,
             "cases_unhealthy_at_end": int(sum(1 for h in res["health"] if h))}
 
 
@@ -342,10 +347,11 @@ def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
         srv.close()
     k = max(1, int(round(sample_frac * n_requests)))
     done = [r for r in reqs if r.error is None]  # requests retired as failed carry no result
-    errs = []
+    errs, floors = [], []
     for r in done[:: max(1, len(done) // k)][:k]:
         ref = SV.alone(r, precision=0)
         errs.append(float(SV.mixture_rel_l2(r.result[0], ref[0]).max()))
+        floors.append(float(SV.mixture_rel_l2(SV.alone(r, precision=0, noise=2.0 ** -24)[0], ref[0]).max()))
     cells = sum(int(r.mesh.iblank.sum()) for r in reqs)
     return {"workload": f"{n_requests} requests, Poisson {rate}/s, boxes n 10..70 ({cells} tets in all), "
                         "50..500 steps, 200 GB budget, smallest work first within 1M resident active cells, "
@@ -353,7 +359,7 @@ def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
             "updates_per_s": st["updates_per_s"], "tts_p50_s": st["tts_p50_s"], "tts_p99_s": st["tts_p99_s"],
             "makespan_s": st["makespan_s"], "gpu_busy_s": st["gpu_busy_s"],
             "peak_resident_gb": st["peak_resident_bytes"] / 1e9, "requests_failed": st["failed"],
-            "mixture_rel_l2_vs_fp64_sample": errs}
+            "mixture_rel_l2_vs_fp64_sample": errs, "mixture_rel_l2_floor_sample": floors}
 
 
 def main():

@@ -130,6 +130,8 @@ const char *hp_last_error(const hp_ctx *ctx);
  *          ntot rows) 8 gradients(33: ug vg wg tg up vp wp tp pg pp phig,
  *          each x y z) 9 phi 10 vis(8) 11 dt/time/total (3 doubles)
  *        12 dl (minimum edge length per cell; read-only)
+ *        13, 14 the FP32-bulk form's gradients (33, order of 8) and limiter,
+ *          widened to double (precision 1 only; read-only)
  * The array is variable-major with leading dimension neles (ntot for 7). */
 int hp_get_array(hp_ctx *ctx, int which, double *host, size_t count);
 int hp_set_array(hp_ctx *ctx, int which, const double *host, size_t count);

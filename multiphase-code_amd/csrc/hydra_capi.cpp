@@ -707,6 +707,18 @@ int hp_get_array(hp_ctx *c, int which, double *host, size_t count)
         host[0] = sc[SC_DT]; host[1] = sc[SC_TIME]; host[2] = tot;
         return HP_OK;
     }
+    if (which == 13 || which == 14) {
+        // the FP32-bulk form's gradients and limiter, widened to double
+        const size_t n = which == 13 ? (size_t)3 * G_N * c->m.ne : (size_t)c->m.ne;
+        const float *src = which == 13 ? c->s.g32 : c->s.phi32;
+        if (!src) return fail(c, HP_ERR_ARG, "hp_get_array: arrays 13/14 exist in the FP32-bulk form only");
+        if (count != n) return fail(c, HP_ERR_ARG, "hp_get_array: count mismatch");
+        std::vector<float> tmp(n);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (size_t k = 0; k < n; k++) host[k] = tmp[k];
+        return HP_OK;
+    }
     double *p;
     size_t n;
     if (int rc = array_view(c, which, &p, &n)) return rc;
@@ -719,7 +731,7 @@ int hp_get_array(hp_ctx *c, int which, double *host, size_t count)
 int hp_set_array(hp_ctx *c, int which, const double *host, size_t count)
 {
     if (!c || !host) return fail(c, HP_ERR_ARG, "hp_set_array: bad argument");
-    if (which == 12) return fail(c, HP_ERR_ARG, "hp_set_array: array 12 (dl) is read-only");
+    if (which >= 12) return fail(c, HP_ERR_ARG, "hp_set_array: arrays 12-14 are read-only");
     double *p;
     size_t n;
     if (int rc = array_view(c, which, &p, &n)) return rc;

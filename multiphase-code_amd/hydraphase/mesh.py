@@ -383,6 +383,29 @@ def sphere_state(mesh: Mesh, p: dict, spheres, *, width_cells: float = 2.0):
     return np.ascontiguousarray(cu), np.ascontiguousarray(gp)
 
 
+def smooth_state(mesh: Mesh, p: dict, *, amp: float = 0.05, center=(0.0, 0.0, 0.0), sigma: float = 0.15):
+    """Restart state without an interface: the air carrier everywhere (gas
+    fraction 1 - phipzero, the inflow's liquid trace), the same velocity and
+    temperature in both phases, and a smooth Gaussian bump of relative
+    height ``amp`` and width ``sigma`` on the pressure and the x velocity.
+    Returns (cu, ghost gas primitives) like sphere_state."""
+    cx = mesh.x[mesh.nod].mean(axis=0)
+    cy = mesh.y[mesh.nod].mean(axis=0)
+    cz = mesh.z[mesh.nod].mean(axis=0)
+    r2 = (cx - center[0]) ** 2 + (cy - center[1]) ** 2 + (cz - center[2]) ** 2
+    bump = amp * np.exp(-r2 / (sigma * sigma))
+    ne = mesh.neles
+    one = np.ones(ne)
+    u = p["ugmin"] * (1.0 + bump)
+    phig = (1.0 - p["phipzero"]) * one
+    cu, rog = prims_to_conserved(p, phig, p["pmin"] * (1.0 + bump), p["tgmin"] * one, u, 0 * one, 0 * one,
+                                 p["tpmin"] * one, u, 0 * one, 0 * one)
+    ngh = mesh.nghosts
+    gp = np.stack([p["ugmin"] * np.ones(ngh), np.zeros(ngh), np.zeros(ngh),
+                   p["pmin"] * np.ones(ngh), p["tgmin"] * np.ones(ngh), rog[mesh.gparent]])
+    return np.ascontiguousarray(cu), np.ascontiguousarray(gp)
+
+
 def sphere_train(seed: int, n: int = 8, rmin: float = 0.03, rmax: float = 0.08,
                  lo=(-0.5, -0.5, -0.5), hi=(0.5, 0.5, 0.5), avoid=None):
     """n spheres with radii U[rmin, rmax], clear of each other and of

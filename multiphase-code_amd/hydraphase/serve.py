@@ -178,14 +178,23 @@ class Server:
                 "tts_p50_s": pct(50), "tts_p99_s": pct(99), "peak_resident_bytes": peak}
 
 
-def alone(r: Request, precision: int | None = None):
+def perturbed(cu, rel: float, seed: int = 2024):
+    """cu with every entry multiplied by 1 + rel * N(0, 1) (seeded): the
+    input noise whose effect on the FP64 step measures the problem's
+    conditioning floor (rel = 2**-24: FP32 unit roundoff)."""
+    rng = np.random.default_rng(seed)
+    return cu * (1.0 + rel * rng.standard_normal(cu.shape))
+
+
+def alone(r: Request, precision: int | None = None, noise: float = 0.0):
     """The request's case run on its own (hp_step), optionally in another
-    precision: the reference the mix is checked against."""
+    precision or from its initial state perturbed by relative noise
+    (perturbed): the reference the mix is checked against."""
     p = dict(r.params)
     if precision is not None:
         p["precision"] = precision
     s = Solver(r.mesh, p)
-    s.upload(r.cu, r.gp, 0.0, 0)
+    s.upload(perturbed(r.cu, noise) if noise else r.cu, r.gp, 0.0, 0)
     s.step(r.spec.steps)
     out = s.download()
     s.close()

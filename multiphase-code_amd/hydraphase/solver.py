@@ -16,7 +16,7 @@ from . import _lib
 from ._lib import HydraError
 
 ARRAY_ID = {"cn": 0, "cu": 1, "cn1": 2, "rhs": 3, "d24": 4, "d2q": 5, "anu": 6, "q": 7, "grad": 8,
-            "phi": 9, "vis": 10, "dl": 12}
+            "phi": 9, "vis": 10, "dl": 12, "grad32": 13, "phi32": 14}
 Q_NAMES = ["ug", "vg", "wg", "pg", "pp", "tg", "phig", "up", "vp", "wp", "phip", "tp", "rop", "rog"]
 G_NAMES = ["ug", "vg", "wg", "tg", "up", "vp", "wp", "tp", "pg", "pp", "phig"]
 
@@ -158,7 +158,7 @@ class Solver:
         ne, nt = self.ne, self.nt
         return {"cn": (10, ne), "cu": (10, ne), "cn1": (10, ne), "rhs": (10, ne), "d24": (10, ne),
                 "d2q": (10, ne), "anu": (2, ne), "q": (14, nt), "grad": (33, ne), "phi": (ne,),
-                "vis": (8, ne), "dl": (ne,)}[name]
+                "vis": (8, ne), "dl": (ne,), "grad32": (33, ne), "phi32": (ne,)}[name]
 
     def get(self, name):
         if name == "scalars":

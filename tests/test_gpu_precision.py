@@ -13,11 +13,21 @@ the reference's step is discontinuous in its input (phase clipping at
 epslnmin, cubic blending band, HLLC wave-branch selection, limiter minima),
 so the FP64 path itself, started from a state perturbed by relative noise of
 FP32 unit roundoff (2^-24), departs from the unperturbed run by a "floor"
-that the FP32-bulk form cannot beat.  The FP32-bulk error must stay within
-FLOOR_FACTOR times that floor.  (A 1e-14 perturbation already moves the
-liquid transverse momenta by ~4e-5 after one step, so a fixed 1e-5 bound on
-every conserved variable is out of reach for any arithmetic that is not
-bit-identical to the oracle.)
+that the FP32-bulk form cannot beat.  The FP32-bulk error of every conserved
+variable (and mixture sum) must stay within FLOOR_FACTOR times that
+variable's own floor, at steps 1, 10 and 100.  (A 1e-14 perturbation already
+moves the liquid transverse momenta by ~4e-5 after one step, so a fixed 1e-5
+bound on every conserved variable of this workload is out of reach for any
+arithmetic that is not bit-identical to the oracle.)
+
+Two tests hold the FP32 form to fixed bounds where the problem allows them:
+a smooth flow without an interface (the 24^3 box, air carrier with its
+liquid trace, a 5% Gaussian bump on pressure and x velocity), where the gas
+mass, x momentum and energy and the mixture mass and energy stay within
+1e-5 relative L2 of the FP64 path at steps 10 and 100; and one stage of the
+routines from one identical state, where the viscous fluxes, gradients,
+limiter and residual of the two forms agree to a few FP32 roundoffs of each
+quantity's scale.
 """
 import hashlib
 import json
@@ -66,8 +76,8 @@ def errors(cu, ref):
     return l2, mx, ml2
 
 
-def compare(myflux, checkpoints, spec=None):
-    m, p, cu0, gp0 = box24_case(myflux)
+def compare(myflux, checkpoints, spec=None, case=None):
+    m, p, cu0, gp0 = case if case is not None else box24_case(myflux)
     ref = run(m, p, cu0, gp0, 0, checkpoints)
     if spec is not None:
         for n, h in ((10, spec["step10"]["cu_sha256"]), (100, spec["cu_sha256"])):
@@ -92,15 +102,100 @@ def compare(myflux, checkpoints, spec=None):
     return rows
 
 
+def assert_within_floor(rows):
+    """Every conserved variable and mixture sum within FLOOR_FACTOR times
+    its own floor."""
+    for n, (e32, efl) in rows.items():
+        for k, v in enumerate(VARS):
+            assert e32[0][k] <= FLOOR_FACTOR * efl[0][k], (n, v, e32[0][k], efl[0][k])
+        for k, v in enumerate(MIX):
+            assert e32[2][k] <= FLOOR_FACTOR * efl[2][k], (n, "mixture " + v, e32[2][k], efl[2][k])
+
+
 def test_fp32_bulk_error_vs_fp64_oracle():
     spec = json.load(open(os.path.join(HERE, "golden", "box24_100steps.json")))
-    rows = compare(2, [10, 100], spec)
-    for n, (e32, efl) in rows.items():
-        assert e32[0].max() <= FLOOR_FACTOR * efl[0].max(), (n, e32[0].max(), efl[0].max())
-        assert e32[2].max() <= FLOOR_FACTOR * efl[2].max(), (n, e32[2].max(), efl[2].max())
+    assert_within_floor(compare(2, [1, 10, 100], spec))
 
 
 def test_fp32_bulk_zien_within_floor():
-    rows = compare(1, [10])
-    e32, efl = rows[10]
-    assert e32[0].max() <= FLOOR_FACTOR * efl[0].max(), (e32[0].max(), efl[0].max())
+    assert_within_floor(compare(1, [1, 10]))
+
+
+SMOOTH_BOUND = 1e-5
+
+
+def test_fp32_bulk_smooth_flow_within_1e5():
+    """No interface, no clipping switch: the FP32-bulk form within 1e-5
+    relative L2 of the FP64 path on the gas mass, x momentum and energy and
+    on the mixture mass and energy, at steps 10 and 100."""
+    m = M.box_mesh(24, 24, 24, jitter=0.1, seed=1)
+    p = M.water_air_params(cfl=0.5)
+    cu0, gp0 = M.smooth_state(m, p, amp=0.01)
+    rows = compare(2, [10, 100], case=(m, p, cu0, gp0))
+    for n, (e32, efl) in rows.items():
+        for k in (0, 1, 4):
+            assert e32[0][k] <= SMOOTH_BOUND, (n, VARS[k], e32[0][k], efl[0][k])
+        for k in (0, 4):
+            assert e32[2][k] <= SMOOTH_BOUND, (n, "mixture " + MIX[k], e32[2][k], efl[2][k])
+
+
+def stage_outputs(m, p, cu, gp, precision):
+    """One stage of the routines from one state: primitive, dt, smoothing,
+    viscous fluxes, then CallFluxSolver (BCs, gradients, limiter, face
+    fluxes, source) leaving the residual in rhs."""
+    from hydraphase import Solver
+    s = Solver(m, dict(p, precision=precision))
+    s.upload(cu, gp, 0.0, 0)
+    s.primitive()
+    dt = s.dt_min() * p["cfl"]
+    s.set_dt(dt, dt, 1)
+    s.smooth()
+    s.viscousgp()
+    s.flux()
+    out = {"vis": s.get("vis"), "rhs": s.get("rhs"),
+           "grad": s.get("grad32" if precision else "grad"), "phi": s.get("phi32" if precision else "phi")}
+    s.close()
+    return out
+
+
+def test_fp32_bulk_one_stage_against_fp64():
+    """The FP32-bulk routines against the FP64 ones from one identical state
+    (the sphere case after 3 FP64 steps, viscous gas): per variable, the
+    largest difference over the active cells relative to that variable's
+    largest magnitude.  Gradients (33 components), viscous fluxes (8) and
+    the residual (10) within STAGE_BOUND; the limiter, which switches on
+    comparisons, within 1e-5 on 99% of the active cells."""
+    from hydraphase import Solver
+    m = M.box_mesh(24, 24, 24, jitter=0.1, seed=1)
+    p = M.water_air_params(cfl=0.5, viscous_gas=True)
+    cu0, gp0 = M.sphere_state(m, p, [(0.0, 0.0, 0.0, 0.2)])
+    s = Solver(m, p)
+    s.upload(cu0, gp0, 0.0, 0)
+    s.step(3)
+    cu, gp, _, _ = s.download()
+    s.close()
+    a64 = stage_outputs(m, p, cu, gp, 0)
+    a32 = stage_outputs(m, p, cu, gp, 1)
+    act = m.iblank == 1
+    worst = {}
+    for name in ("grad", "vis"):
+        r64, r32 = a64[name][:, act], a32[name][:, act]
+        scale = np.abs(r64).max(axis=1)
+        worst[name] = np.abs(r32 - r64).max(axis=1) / np.maximum(scale, 1e-300)
+    # residuals against their family's scale: each phase's mass, momentum
+    # (largest of its three components) and energy
+    r64, r32 = a64["rhs"][:, act], a32["rhs"][:, act]
+    mag = np.abs(r64).max(axis=1)
+    fam = np.array([mag[0], *[mag[1:4].max()] * 3, mag[4], mag[5], *[mag[6:9].max()] * 3, mag[9]])
+    worst["rhs"] = np.abs(r32 - r64).max(axis=1) / np.maximum(fam, 1e-300)
+    for name, rel in worst.items():
+        print(f"{name}: max relative difference per variable {np.array2string(rel, precision=2)}")
+    dphi = np.abs(a32["phi"][act] - a64["phi"][act])
+    q99 = np.quantile(dphi, 0.99)
+    print(f"phi: median {np.median(dphi):.2e}, 99% {q99:.2e}, max {dphi.max():.2e}")
+    assert (worst["grad"] <= 2e-6).all() and (worst["vis"] <= 2e-6).all()
+    assert (worst["rhs"][:5] <= 2e-5).all() and (worst["rhs"][5:] <= STAGE_RHS_LIQUID).all()
+    assert q99 <= 1e-6 and dphi.max() <= 1e-5
+
+
+STAGE_RHS_LIQUID = 1e-4
