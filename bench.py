@@ -49,6 +49,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "multiphase-code_amd"))
 
 HBM_PEAK_GBS = 8000.0
+FLUX_TIMING_STEPS = 5
 # algorithmic bytes of one face-flux launch per active cell (FP64): own
 # primitives 14x8, gradients 33x8, limiter 8, area vectors 12x8, face
 # centroids 12x8, centroid 3x8, neighbour ids 4x4, vol 8, interphase source
@@ -415,15 +416,23 @@ def main():
     if args.warmup:
         s.step(args.warmup)
         s.synchronize()
+    # the timed steps: hp_step replays one captured step (HIP graph)
     s.step(args.steps)
     s.synchronize()
-    total_ms, flux_ms, nflux = s.last_timing()
+    total_ms, _, _ = s.last_timing()
+    # after them, a few eager steps with HIP events around every flux
+    # launch for the roofline figures
+    s.set_flux_timing(True)
+    s.step(FLUX_TIMING_STEPS)
+    s.synchronize()
+    _, flux_ms, nflux = s.last_timing()
+    s.set_flux_timing(False)
     cu_out, _, tphys, tot = s.download()
     if not np.isfinite(cu_out).all():
         print("non-finite state after the timed steps", file=sys.stderr)
         sys.exit(1)
     active = int((m.iblank == 1).sum())
-    updates = active * 4 * args.steps
+    updates = active * 4 * args.steps  # the timed steps only
     ms_per_step = total_ms / args.steps
     value = updates / (total_ms * 1e-3)
     flux_avg_ms = flux_ms / max(nflux, 1)

@@ -103,8 +103,18 @@ int hp_rk(hp_ctx *ctx);               /* sr_RK.F90:5-168 */
  * 2,3,4,1 into diffs (may be NULL: then no host sync), cu=cn. */
 int hp_end_step(hp_ctx *ctx, double *diffs);
 /* nsteps full time steps (main.F90:515-625) with the time step reduced and
- * applied on the device; no host synchronisation. */
+ * applied on the device, each ending with the max|cn-cu| monitor of
+ * variables 1..4 (main.F90:603-612) reduced on the device; no host
+ * synchronisation.  Replays one captured HIP graph of a step unless per-flux
+ * timing is on (hp_set_flux_timing). */
 int hp_step(hp_ctx *ctx, int nsteps);
+/* hp_step that also returns every step's monitor: diffs(4, nsteps) =
+ * diffu, diffv, diffw, diffp per step, the values main.F90:653 prints (one
+ * device-to-host copy at the end; synchronises). */
+int hp_step_diffs(hp_ctx *ctx, int nsteps, double *diffs);
+/* the monitor of the context's last fused step (hp_step, or a batch step
+ * of its batch): diffu, diffv, diffw, diffp (synchronises) */
+int hp_last_diffs(hp_ctx *ctx, double *diffs);
 
 /* The 14 primitives (order of hp_get_array id 7) of one cell, 1-based id;
  * the driver's Velocity.dat probe |u_g(1)| (main.F90:640-642, :654). */
@@ -137,8 +147,14 @@ int hp_get_array(hp_ctx *ctx, int which, double *host, size_t count);
 int hp_set_array(hp_ctx *ctx, int which, const double *host, size_t count);
 
 /* Timing of the last hp_step() call, measured with HIP events on the
- * context's stream: total milliseconds and the flux kernel's share. */
+ * context's stream: total milliseconds and, when hp_set_flux_timing is on,
+ * the flux kernel's share (else 0 launches).  No counterpart in the
+ * reference. */
 int hp_last_timing(hp_ctx *ctx, double *total_ms, double *flux_ms, int *flux_launches);
+/* on != 0: hp_step brackets every flux launch with HIP events (for the
+ * roofline figures; the step then runs eagerly instead of as a graph).
+ * Off by default. */
+int hp_set_flux_timing(hp_ctx *ctx, int on);
 
 /* Overset coupling by fixed donor lists.  No counterpart in the reference,
  * which only freezes iblank cells (its overset preprocessing is an

@@ -84,9 +84,16 @@ struct hp_ctx {
     int *health = nullptr;      // device flag word of hp_health
     std::string err;
     // timing of the last hp_step
-    std::vector<hipEvent_t> fev;  // start/stop pairs around flux launches
+    std::vector<hipEvent_t> fev;  // start/stop pairs around flux launches (hp_set_flux_timing)
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int nflux_timed = 0;
+    bool flux_timing = false;
+    // per-step monitor maxima of hp_step_diffs (4 per step, device)
+    double *mon_log = nullptr;
+    int mon_cap = 0;
+    // one captured time step for hp_step (rebuilt when the donor lists change)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -103,6 +110,18 @@ int fail(hp_ctx *c, int code, const std::string &msg)
 }  // namespace
 
 int hydra::set_global_error(int code, const std::string &msg) { return fail(nullptr, code, msg); }
+
+// forget the captured time step of hp_step (its launch arguments changed)
+static int drop_step_graph(hp_ctx *c)
+{
+    hipError_t e = hipSuccess;
+    if (c->exec) e = hipGraphExecDestroy(c->exec);
+    if (c->graph && e == hipSuccess) e = hipGraphDestroy(c->graph);
+    c->exec = nullptr;
+    c->graph = nullptr;
+    if (e != hipSuccess) return fail(c, HP_ERR_HIP, std::string("graph release: ") + hipGetErrorString(e));
+    return HP_OK;
+}
 
 namespace {
 
@@ -205,8 +224,10 @@ int ts_upload(TileSet &t, hipStream_t st, std::string &err)
 }
 
 // one time step (main.F90:515-625) of every case in B; fev (8 events, may
-// be null) brackets the four flux launches
-void enqueue_step(const Batch &B, const Grid &G, hipStream_t st, hipEvent_t *fev, const Donors *D = nullptr)
+// be null) brackets the four flux launches; the step's monitor maxima go to
+// mon_out (4 doubles) when it is not null, else to each slot's State::mon
+void enqueue_step(const Batch &B, const Grid &G, hipStream_t st, hipEvent_t *fev, const Donors *D = nullptr,
+                  double *mon_out = nullptr)
 {
     launch_eos(B, G, 1, st);       // primitive + CFL bound per tile
     launch_dt_final(B, G, st);     // dt, time, total per case
@@ -216,7 +237,8 @@ void enqueue_step(const Batch &B, const Grid &G, hipStream_t st, hipEvent_t *fev
     launch_eos(B, G, 0, st);       // stage 1's primitiveplusbc
     // Each stage's flux launch ends with the primitive recovery the next
     // stage starts with (the step's last one, main.F90:595, with cu = cn,
-    // :625; no monitor diffs here), written to the other primitive buffer:
+    // :625, and the monitor differences, :603-612), written to the other
+    // primitive buffer:
     // stages 1 and 3 read q, stages 2 and 4 read q2 (Bs), and the step ends
     // in q, where it started.
     Batch Bs = B;
@@ -235,6 +257,7 @@ void enqueue_step(const Batch &B, const Grid &G, hipStream_t st, hipEvent_t *fev
             launch_donor_gather(B.P1, s, *D, k == 4 ? 2 : 1, st);
         }
     }
+    launch_monitor_final(B, G, D, mon_out, st);  // max|cn - cu| of variables 1..4
 }
 
 }  // namespace
@@ -343,7 +366,7 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     ALLOC(s.cn, 10 * (size_t)ne) ALLOC(s.cu, 10 * (size_t)ne) ALLOC(s.cn1, 10 * (size_t)ne)
     ALLOC(s.q, (size_t)Q_N * ntot) ALLOC(s.q2, (size_t)Q_N * ntot) ALLOC(s.grad, (size_t)3 * G_N * ne) ALLOC(s.phi, ne)
     ALLOC(s.vis, 8 * (size_t)ne) ALLOC(s.d24, 10 * (size_t)ne) ALLOC(s.d2q, 10 * (size_t)ne)
-    ALLOC(s.anu, 2 * (size_t)ne) ALLOC(s.rhs, 10 * (size_t)ne) ALLOC(s.scal, SC_N) ALLOC(s.itot, 1)
+    ALLOC(s.anu, 2 * (size_t)ne) ALLOC(s.rhs, 10 * (size_t)ne) ALLOC(s.scal, SC_N) ALLOC(s.itot, 1) ALLOC(s.mon, 4)
     ALLOC(c->health, 1)
     if (params->precision == 1) {
         ALLOC(s.g32, (size_t)3 * G_N * ne) ALLOC(s.phi32, ne)
@@ -416,6 +439,9 @@ int hp_destroy(hp_ctx *c)
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (void *p : c->bufs) (void)hipFree(p);
     for (hipEvent_t e : c->fev) (void)hipEventDestroy(e);
+    if (c->exec) (void)hipGraphExecDestroy(c->exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->mon_log) (void)hipFree(c->mon_log);
     if (c->t0) (void)hipEventDestroy(c->t0);
     if (c->t1) (void)hipEventDestroy(c->t1);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -609,29 +635,88 @@ int hp_end_step(hp_ctx *c, double *diffs)
     return HP_OK;
 }
 
-int hp_step(hp_ctx *c, int nsteps)
+int hp_step(hp_ctx *c, int nsteps) { return hp_step_diffs(c, nsteps, nullptr); }
+
+int hp_step_diffs(hp_ctx *c, int nsteps, double *diffs)
 {
     if (!c || nsteps < 0) return fail(c, HP_ERR_ARG, "hp_step: bad argument");
-    const int need = 8 * nsteps;
-    while ((int)c->fev.size() < need) {
-        hipEvent_t e;
-        HIPCHK(c, hipEventCreate(&e));
-        c->fev.push_back(e);
+    if (c->flux_timing) {
+        const int need = 8 * nsteps;
+        while ((int)c->fev.size() < need) {
+            hipEvent_t e;
+            HIPCHK(c, hipEventCreate(&e));
+            c->fev.push_back(e);
+        }
+    }
+    if (diffs && nsteps > c->mon_cap) {
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        if (c->mon_log) HIPCHK(c, hipFree(c->mon_log));
+        c->mon_log = nullptr;
+        c->mon_cap = 0;
+        HIPCHK(c, hipMalloc(&c->mon_log, sizeof(double) * 4 * (size_t)nsteps));
+        c->mon_cap = nsteps;
     }
     if (!c->t0) {
         HIPCHK(c, hipEventCreate(&c->t0));
         HIPCHK(c, hipEventCreate(&c->t1));
     }
     if (int rc = sync_owner(c)) return rc;
+    const Donors *D = c->D.n ? &c->D : nullptr;
+    // The plain step (no per-flux events, monitor into State::mon) replays
+    // one captured graph: kernel launches only, every argument fixed for
+    // the context's lifetime except the donor lists, whose change drops it.
+    const bool graph = !c->flux_timing && !diffs && nsteps > 0;
+    if (graph && !c->exec) {
+        HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        enqueue_step(c->ts.view(), c->G, c->st, nullptr, D, nullptr);
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamEndCapture(c->st, &g));
+        c->graph = g;
+        HIPCHK(c, hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
+    }
     HIPCHK(c, hipEventRecord(c->t0, c->st));
     int ev = 0;
     for (int n = 0; n < nsteps; n++) {
-        enqueue_step(c->ts.view(), c->G, c->st, &c->fev[ev], c->D.n ? &c->D : nullptr);
-        ev += 8;
+        if (graph) {
+            HIPCHK(c, hipGraphLaunch(c->exec, c->st));
+            continue;
+        }
+        enqueue_step(c->ts.view(), c->G, c->st, c->flux_timing ? &c->fev[ev] : nullptr, D,
+                     diffs ? c->mon_log + 4 * (size_t)n : nullptr);
+        if (c->flux_timing) ev += 8;
     }
     HIPCHK(c, hipEventRecord(c->t1, c->st));
     c->nflux_timed = ev / 2;
+    if (diffs && nsteps > 0) {
+        std::vector<double> h(4 * (size_t)nsteps);
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->mon_log, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        // main.F90:653 prints diffu, diffv, diffw, diffp = variables 2,3,4,1
+        for (int n = 0; n < nsteps; n++) {
+            const double *d = &h[4 * (size_t)n];
+            diffs[4 * (size_t)n] = d[1]; diffs[4 * (size_t)n + 1] = d[2];
+            diffs[4 * (size_t)n + 2] = d[3]; diffs[4 * (size_t)n + 3] = d[0];
+        }
+    }
     return finish(c, "step");
+}
+
+int hp_last_diffs(hp_ctx *c, double *diffs)
+{
+    if (!c || !diffs) return fail(c, HP_ERR_ARG, "hp_last_diffs: bad argument");
+    if (int rc = sync_owner(c)) return rc;
+    double d[4];
+    HIPCHK(c, hipMemcpyAsync(d, c->s.mon, sizeof(d), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    diffs[0] = d[1]; diffs[1] = d[2]; diffs[2] = d[3]; diffs[3] = d[0];
+    return HP_OK;
+}
+
+int hp_set_flux_timing(hp_ctx *c, int on)
+{
+    if (!c) return fail(nullptr, HP_ERR_ARG, "null context");
+    c->flux_timing = on != 0;
+    return HP_OK;
 }
 
 int hp_last_timing(hp_ctx *c, double *total_ms, double *flux_ms, int *flux_launches)
@@ -769,6 +854,7 @@ int hp_set_donors(hp_ctx *c, int n, const int *recv, const int *donors, int ld, 
         hw[2 * (size_t)r + 1] = make_double2(weights[(size_t)2 * ld + r], weights[(size_t)3 * ld + r]);
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
+    if (int rc = drop_step_graph(c)) return rc;
     for (void *p : c->dbufs) HIPCHK(c, hipFree(p));
     c->dbufs.clear();
     c->D = Donors{};
@@ -777,11 +863,12 @@ int hp_set_donors(hp_ctx *c, int n, const int *recv, const int *donors, int ld, 
     c->ts.hS[0].dflag = nullptr;
     c->ts.hS[0].cnd = nullptr;
     if (n == 0) return ts_upload(c->ts, c->st, c->err);
-    // a per-cell donor flag, and where the fused step's flux keeps the
-    // donors' stage update for the gather
+    // a per-cell flag (bit 0 donor, bit 1 receptor), and where the fused
+    // step's flux keeps the donors' stage update for the gather
     std::vector<unsigned char> hflag(ne, 0);
     for (int r = 0; r < n; r++) {
-        hflag[hid[r].x] = 1; hflag[hid[r].y] = 1; hflag[hid[r].z] = 1; hflag[hid[r].w] = 1;
+        hflag[hid[r].x] |= 1; hflag[hid[r].y] |= 1; hflag[hid[r].z] |= 1; hflag[hid[r].w] |= 1;
+        hflag[hr[r]] |= 2;
     }
     void *pf = nullptr, *pd = nullptr;
     HIPCHK(c, hipMalloc(&pf, ne));
@@ -796,6 +883,9 @@ int hp_set_donors(hp_ctx *c, int n, const int *recv, const int *donors, int ld, 
     c->dbufs.push_back(pi);
     HIPCHK(c, hipMalloc(&pw, sizeof(double2) * 2 * (size_t)n));
     c->dbufs.push_back(pw);
+    void *pp = nullptr;  // the gather's monitor partials, 4 per block of receptors
+    HIPCHK(c, hipMalloc(&pp, sizeof(double) * 4 * (size_t)((n + DONOR_BLOCK - 1) / DONOR_BLOCK)));
+    c->dbufs.push_back(pp);
     HIPCHK(c, hipMemcpyAsync(pr, hr.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(pi, hid.data(), sizeof(int4) * n, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(pw, hw.data(), sizeof(double2) * 2 * (size_t)n, hipMemcpyHostToDevice, c->st));
@@ -804,6 +894,7 @@ int hp_set_donors(hp_ctx *c, int n, const int *recv, const int *donors, int ld, 
     c->D.recv = static_cast<const int *>(pr);
     c->D.ids = static_cast<const int4 *>(pi);
     c->D.w = static_cast<const double2 *>(pw);
+    c->D.part = static_cast<double *>(pp);
     c->s.dflag = static_cast<const unsigned char *>(pf);
     c->s.cnd = static_cast<double *>(pd);
     c->ts.hS[0].dflag = c->s.dflag;
@@ -1092,7 +1183,7 @@ size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision)
     size_t b = 3 * (size_t)nodes * D;                        // coordinates
     b += (13 * ne + 4 * ng) * I;                             // nod, nc, ncp, iblank, ghost lists
     b += (12 + 3 + 12 + 12) * ne * D + 4 * nt * D;         // sc, dl/epssq/dd, fc0, fc3, xc..vol
-    b += (30 + 3 * G_N + 1 + 8 + 10 + 10 + 2 + 10) * ne * D + 2 * (size_t)Q_N * nt * D + SC_N * D + I;
+    b += (30 + 3 * G_N + 1 + 8 + 10 + 10 + 2 + 10) * ne * D + 2 * (size_t)Q_N * nt * D + (SC_N + 4) * D + I;
     if (precision == 1) b += (3 * G_N + 1) * ne * sizeof(float);  // FP32 gradients, limiter
     else b += SRC_N * ne * D;                                     // interphase source terms
     const size_t tc = (ne + TILE - 1) / TILE, tg = (ng + TILE - 1) / TILE + 1;

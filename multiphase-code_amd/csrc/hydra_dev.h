@@ -74,8 +74,9 @@ struct State {
     const double *dl;       // ne (same as Mesh::dl)
     double *scal;           // SC_N
     int *itot;              // total (step counter used by the outflow BC)
-    const unsigned char *dflag;  // ne: 1 at overset donors (null without donor lists)
+    const unsigned char *dflag;  // ne: bit 0 overset donor, bit 1 overset receptor (null without donor lists)
     double *cnd;            // 10 x ne: the donors' stage update for the fused step's gather (with donor lists)
+    double *mon;            // 4: the last fused step's max|cn - cu| of variables 1..4 (main.F90:603-612)
 };
 
 // Work of one launch.  Cells (and boundary ghosts) of every case are cut
@@ -145,7 +146,14 @@ struct Donors {
     const int *recv;      // n
     const int4 *ids;      // n, 0-based
     const double2 *w;     // 2n: (w1, w2), (w3, w4)
+    double *part;         // 4 per block of 256 receptors: the receptors' monitor maxima at the step's end
 };
+constexpr int DONOR_BLOCK = 256;
+// the monitor maxima of each slot's fused step (main.F90:603-612) from the
+// last flux launch's tile partials (and, for a lone case, the last donor
+// gather's): into out[0..3] when out is not null, else into the slot's
+// State::mon
+void launch_monitor_final(const Batch &B, const Grid &G, const Donors *D, double *out, hipStream_t st);
 // eos: 0 the gather from cn alone; fused step, 1 the gather from cnd, then
 // the receptors' primitive recovery into s.q, 2 as 1 plus cu = cn (last stage)
 void launch_donor_gather(const Params &P, const State &s, const Donors &D, int eos, hipStream_t st);

@@ -1180,7 +1180,8 @@ __device__ __forceinline__ void source_and_update(const State &s, const Mesh &m,
 // cn, and in cu as well at the end of the step (`end`).  An overset donor
 // also keeps its stage update, unrewritten, in cnd: that is what the donor
 // gather after this launch interpolates.
-__device__ __forceinline__ void eos_staged(const Params &P, const State &s, int i, const double *c_lds, bool end)
+__device__ __forceinline__ void eos_staged(const Params &P, const State &s, int i, const double *c_lds, bool end,
+                                           double (&d)[4])
 {
     double c[10], v[Q_N], nv[10];
     for (int k = 0; k < 10; k++) c[k] = c_lds[k * TILE];
@@ -1188,11 +1189,17 @@ __device__ __forceinline__ void eos_staged(const Params &P, const State &s, int 
     const int ne = s.ne;
     const size_t nt = s.nt;
     for (int k = 0; k < Q_N; k++) s.q2[(size_t)k * nt + i] = v[k];
-    if (s.dflag && s.dflag[i])
+    const unsigned char fl = s.dflag ? s.dflag[i] : 0;
+    if (fl & 1)
         for (int k = 0; k < 10; k++) s.cnd[(size_t)k * ne + i] = c[k];
     for (int k = 0; k < 10; k++) s.cn[(size_t)k * ne + i] = nv[k];
-    if (end)
+    if (end && !(fl & 2)) {
+        // the monitor (main.F90:603-612) against the step's starting cu, and
+        // cu = cn; an overset receptor's final value, its monitor difference
+        // and its cu come from the gather that follows, which reads its cu
+        for (int k = 0; k < 4; k++) d[k] = fabs(nv[k] - s.cu[(size_t)k * ne + i]);
         for (int k = 0; k < 10; k++) s.cu[(size_t)k * ne + i] = nv[k];
+    }
 }
 
 // One cell per quad of lanes, one face per lane (sr_hllcbatten.F90:77-555):
@@ -2178,8 +2185,16 @@ __global__ void __launch_bounds__(4 * TILE) k_flux(Batch B, int stage, int feos)
         }
         if (feos) {
             __syncthreads();
+            double d[4] = {0.0, 0.0, 0.0, 0.0};
             if ((int)threadIdx.x < C.count)
-                eos_staged(C.P, C.S, C.first + (int)threadIdx.x, stage_lds + threadIdx.x, feos == 2);
+                eos_staged(C.P, C.S, C.first + (int)threadIdx.x, stage_lds + threadIdx.x, feos == 2, d);
+            if (feos == 2 && threadIdx.x < TILE) {
+                // the tile's monitor maxima (a max does not depend on the order)
+                for (int k = 0; k < 4; k++) {
+                    const double v = wave_max(d[k]);
+                    if (threadIdx.x == 0) B.part[(size_t)4 * t + k] = v;
+                }
+            }
             if (MODE == 1 && t + st_ < e_) __syncthreads();  // the next tile reuses the staging
         }
     }
@@ -2344,28 +2359,83 @@ void launch_health(const Batch &B, const Grid &G, int *flags, hipStream_t st)
 // overset: each receptor's conserved variables from its four donors,
 // w1*q1 + w2*q2 + w3*q3 + w4*q4 in that order (receptors are never donors,
 // so the reads never see this launch's writes)
-__global__ void __launch_bounds__(256) k_donor_gather(Params P, State s, Donors D, int eos)
+__global__ void __launch_bounds__(DONOR_BLOCK) k_donor_gather(Params P, State s, Donors D, int eos)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= D.n) return;
-    const int4 d = D.ids[r];
-    const double2 wa = D.w[2 * r], wb = D.w[2 * r + 1];
-    const int i = D.recv[r], ne = s.ne;
-    // fused step: the donors' stage update as the flux launch kept it (cnd);
-    // their cn already holds the rewrite of their primitive recovery
-    const double *src = eos ? s.cnd : s.cn;
-    for (int k = 0; k < 10; k++) {
-        const double *c = src + (size_t)k * ne;
-        s.cn[(size_t)k * ne + i] = wa.x * c[d.x] + wa.y * c[d.y] + wb.x * c[d.z] + wb.y * c[d.w];
+    double dm[4] = {0.0, 0.0, 0.0, 0.0};
+    if (r < D.n) {
+        const int4 d = D.ids[r];
+        const double2 wa = D.w[2 * r], wb = D.w[2 * r + 1];
+        const int i = D.recv[r], ne = s.ne;
+        // fused step: the donors' stage update as the flux launch kept it (cnd);
+        // their cn already holds the rewrite of their primitive recovery
+        const double *src = eos ? s.cnd : s.cn;
+        for (int k = 0; k < 10; k++) {
+            const double *c = src + (size_t)k * ne;
+            s.cn[(size_t)k * ne + i] = wa.x * c[d.x] + wa.y * c[d.y] + wb.x * c[d.z] + wb.y * c[d.w];
+        }
+        // fused step: the receptor's primitive recovery into s.q (the next
+        // stage's buffer), with cu = cn at the end of the step (eos == 2)
+        // and the receptor's monitor differences against the old cu
+        double cu0[4];
+        if (eos == 2)
+            for (int k = 0; k < 4; k++) cu0[k] = s.cu[(size_t)k * ne + i];
+        if (eos) eos_cell(P, s.cn, ne, i, s.q, s.nt, nullptr, s.dl, eos == 2 ? s.cu : nullptr);
+        if (eos == 2)
+            for (int k = 0; k < 4; k++) dm[k] = fabs(s.cn[(size_t)k * ne + i] - cu0[k]);
     }
-    // fused step: the receptor's primitive recovery into s.q (the next
-    // stage's buffer), with cu = cn at the end of the step (eos == 2)
-    if (eos) eos_cell(P, s.cn, ne, i, s.q, s.nt, nullptr, s.dl, eos == 2 ? s.cu : nullptr);
+    if (eos == 2) {
+        __shared__ double red[4][DONOR_BLOCK / 64];
+        for (int k = 0; k < 4; k++) {
+            const double v = wave_max(dm[k]);
+            if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            double v = red[threadIdx.x][0];
+            for (int w = 1; w < DONOR_BLOCK / 64; w++) v = dmax(v, red[threadIdx.x][w]);
+            D.part[(size_t)4 * blockIdx.x + threadIdx.x] = v;
+        }
+    }
 }
 
 void launch_donor_gather(const Params &P, const State &s, const Donors &D, int eos, hipStream_t st)
 {
-    if (D.n > 0) hipLaunchKernelGGL(k_donor_gather, dim3((D.n + 255) / 256), dim3(256), 0, st, P, s, D, eos);
+    if (D.n > 0)
+        hipLaunchKernelGGL(k_donor_gather, dim3((D.n + DONOR_BLOCK - 1) / DONOR_BLOCK), dim3(DONOR_BLOCK), 0, st,
+                           P, s, D, eos);
+}
+
+// per case slot (one block each): the monitor maxima of variables 1..4
+// over the slot's tiles (and a lone case's receptor blocks)
+__global__ void __launch_bounds__(1024) k_monitor_final(Batch B, const double *dpart, int ndpart, double *out)
+{
+    const int c = blockIdx.x;
+    const int t0 = B.slot_t0[c], t1 = B.slot_t1[c];
+    if (t0 >= t1) return;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += (int)blockDim.x)
+        for (int k = 0; k < 4; k++) v[k] = dmax(v[k], B.part[(size_t)4 * t + k]);
+    for (int b = (int)threadIdx.x; b < ndpart; b += (int)blockDim.x)
+        for (int k = 0; k < 4; k++) v[k] = dmax(v[k], dpart[(size_t)4 * b + k]);
+    __shared__ double red[4][16];
+    for (int k = 0; k < 4; k++) {
+        const double w = wave_max(v[k]);
+        if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double r = red[threadIdx.x][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = dmax(r, red[threadIdx.x][w]);
+        double *dst = out ? out : B.S[c].mon;
+        dst[threadIdx.x] = r;
+    }
+}
+
+void launch_monitor_final(const Batch &B, const Grid &G, const Donors *D, double *out, hipStream_t st)
+{
+    const int nd = (D && D->n > 0) ? (D->n + DONOR_BLOCK - 1) / DONOR_BLOCK : 0;
+    hipLaunchKernelGGL(k_monitor_final, dim3(G.slots), dim3(1024), 0, st, B, nd ? D->part : nullptr, nd, out);
 }
 
 }  // namespace hydra

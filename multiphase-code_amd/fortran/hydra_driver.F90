@@ -8,9 +8,10 @@
 ! (:641,654); at the end it writes restart_files/restart0000.in (:812-828)
 ! and tecplot_files/tecplot0000.dat (:956-1266).
 !
-! HYDRA_FUSED=1 in the environment runs each outer iteration as one
-! device-resident hp_step call (no per-step host synchronisation, no
-! per-step monitor output).
+! HYDRA_FUSED=1 in the environment runs each step as the fused,
+! device-resident hp_step_diffs call (the time step reduced and applied on
+! the device, the monitor reduced there), then reads back the monitor, the
+! time and cell 1's primitives for the same two outputs.
 program hydra_driver
   use iso_c_binding
   use hydra_hip_mod
@@ -22,7 +23,7 @@ program hydra_driver
   integer :: iter, iters, niter, itersub, istart, myflux, myorder, u, ios, dummy, g_id
   real(dp), allocatable :: x(:), y(:), z(:), cu(:, :), gp(:, :)
   integer, allocatable :: nod(:, :), nc(:, :), iblank(:), nparent(:), nghost(:), ntype(:), nside(:)
-  real(dp) :: time, dt, dtmin, diffs(4), pr(14), vel_mag
+  real(dp) :: time, dt, dtmin, diffs(4), pr(14), vel_mag, scal(3)
   real(dp) :: rogmin, ropmin, egtotalmin, eptotalmin, phig0, phip0
   character(len=16) :: fused_env
   logical :: fused
@@ -109,8 +110,16 @@ program hydra_driver
   open(unit=23, file='Velocity.dat', access='append')
   do iter = 1, niter
     if (fused) then
-      call hp_check(hp_step(ctx, itersub), ctx, 'hp_step')
-      total = total + itersub
+      do iters = 1, itersub
+        call hp_check(hp_step_diffs(ctx, 1, diffs), ctx, 'hp_step_diffs')
+        total = total + 1
+        call hp_check(hp_get_array(ctx, 11, scal, 3_c_size_t), ctx, 'scalars')  ! dt, time, total
+        time = scal(2)
+        call hp_check(hp_probe_cell(ctx, 1, pr), ctx, 'probe')
+        vel_mag = sqrt(pr(1)**2 + pr(2)**2 + pr(3)**2)
+        print '(I6, 1PE20.10, 5F24.16)', iters, time, diffs(1), diffs(2), diffs(3), diffs(4)
+        write(23, *) time, vel_mag
+      end do
       cycle
     end if
     do iters = 1, itersub

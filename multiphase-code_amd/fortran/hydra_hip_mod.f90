@@ -95,6 +95,28 @@ module hydra_hip_mod
       type(c_ptr), value :: ctx
       integer(c_int), value :: nsteps
     end function
+    integer(c_int) function hp_step_diffs(ctx, nsteps, diffs) bind(C, name="hp_step_diffs")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+      real(c_double), intent(out) :: diffs(4, *)
+    end function
+    integer(c_int) function hp_get_array(ctx, which, host, count) bind(C, name="hp_get_array")
+      import :: c_ptr, c_int, c_double, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which
+      real(c_double), intent(out) :: host(*)
+      integer(c_size_t), value :: count
+    end function
+    ! CFL reduction over device arrays the caller owns, allocating nothing
+    integer(c_int) function hp_dt_min_noalloc(q, ldq, dl, neles, gamag, gamap, pinf, dtmin, stream) &
+        bind(C, name="hp_dt_min_noalloc")
+      import :: c_ptr, c_int, c_double, c_size_t
+      type(c_ptr), value :: q, dl, dtmin, stream
+      integer(c_size_t), value :: ldq
+      integer(c_int), value :: neles
+      real(c_double), value :: gamag, gamap, pinf
+    end function
     integer(c_int) function hp_probe_cell(ctx, cell, prims) bind(C, name="hp_probe_cell")
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

@@ -50,7 +50,7 @@ EXPORTS = ["hp_create", "hp_destroy", "hp_upload_state", "hp_download_state", "h
            "hp_batch_create", "hp_batch_destroy", "hp_batch_add", "hp_batch_remove", "hp_batch_set_grid",
            "hp_batch_step", "hp_batch_synchronize", "hp_batch_last_ms", "hp_batch_last_error",
            "hp_case_bytes", "hp_set_donors", "hp_apply_donors", "hp_dt_min_noalloc",
-           "hp_stream_bandwidth"]
+           "hp_stream_bandwidth", "hp_step_diffs", "hp_last_diffs", "hp_set_flux_timing"]
 
 
 def lib():
@@ -75,6 +75,9 @@ def lib():
     L.hp_rk_stage.argtypes = [vp, c_int]
     L.hp_end_step.argtypes = [vp, dp]
     L.hp_step.argtypes = [vp, c_int]
+    L.hp_step_diffs.argtypes = [vp, c_int, dp]
+    L.hp_last_diffs.argtypes = [vp, dp]
+    L.hp_set_flux_timing.argtypes = [vp, c_int]
     L.hp_last_error.argtypes = [vp]
     L.hp_last_error.restype = ctypes.c_char_p
     L.hp_get_array.argtypes = [vp, c_int, dp, c_size]

@@ -122,6 +122,21 @@ class Solver:
     def step(self, n=1):
         self._chk(self.L.hp_step(self.h, int(n)))
 
+    def step_diffs(self, n=1):
+        """n fused steps; returns each step's monitor (n, 4): diffu, diffv,
+        diffw, diffp (main.F90:653)."""
+        d = np.zeros((int(n), 4))
+        self._chk(self.L.hp_step_diffs(self.h, int(n), _dp(d)))
+        return d
+
+    def last_diffs(self):
+        d = np.zeros(4)
+        self._chk(self.L.hp_last_diffs(self.h, _dp(d)))
+        return d
+
+    def set_flux_timing(self, on=True):
+        self._chk(self.L.hp_set_flux_timing(self.h, 1 if on else 0))
+
     def synchronize(self):
         self._chk(self.L.hp_synchronize(self.h))
 

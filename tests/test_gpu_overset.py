@@ -45,15 +45,19 @@ def test_overset_steps_match_oracle(case, flux):
     s.upload(cu, gp, 0.0, 0)
     s.set_donors(recv, donors, w)
     s.apply_donors()
-    s.step(3)
+    s.step(1)                   # the captured step
+    d1 = s.last_diffs()
+    d23 = s.step_diffs(2)       # eager steps returning their monitor
     cu_g, gp_g, t_g, n_g = s.download()
     s.close()
     o = O.Oracle(m, p)
     o.set_state(cu, gp, 0.0, 0)
     o.set_donors(recv, donors, w)
     o.call("donor_gather")
-    o.run(3)
+    want = [o.step() for _ in range(3)]
     cu_o, gp_o, t_o, n_o = o.get_state()
+    # the monitor (main.F90:603-612) over every cell, receptors included
+    assert np.array_equal(d1, want[0]) and np.array_equal(d23, np.array(want[1:]))
     assert n_g == n_o == 3 and t_g == t_o
     assert np.isfinite(cu_g).all()
     assert np.array_equal(cu_g, cu_o)

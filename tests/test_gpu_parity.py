@@ -196,6 +196,47 @@ def test_steps_bitwise(flux, fused):
 
 
 @pytest.mark.parametrize("flux", [2, 1])
+def test_graph_replays_timing_and_monitor(flux):
+    """hp_step replays one captured step: chunks of 1 and 2 replays with
+    host reads in between, then eager steps with per-flux timing and with the
+    per-step monitor, all equal the oracle step for step, and the device
+    monitor of each step equals the oracle's max|cn-cu| (main.F90:603-612).
+    Five steps: this case's reference step breaks down (NaN) at step 6 with
+    HLLC-Zien, where a maximum over NaN depends on the order it is taken in
+    (as Fortran's MAX with a NaN argument is processor dependent)."""
+    m, p, o = small_case(flux=flux, steps=0)
+    s = solver_cls()(m, p)
+    cu0, gp0, t0, tot0 = o.get_state()
+    s.upload(cu0, gp0, t0, tot0)
+    done = 0
+    for chunk in (1, 2):
+        s.step(chunk)
+        want = [o.step() for _ in range(chunk)]
+        done += chunk
+        got = s.download()
+        ref = o.get_state()
+        assert got[3] == ref[3] == done and got[2] == ref[2]
+        assert_same(f"cu after {done}", got[0], ref[0])
+        assert_same(f"ghosts after {done}", got[1], ref[1])
+        assert np.array_equal(s.last_diffs(), want[-1]), (done, s.last_diffs(), want[-1])
+    s.set_flux_timing(True)
+    s.step(1)
+    want = o.step()
+    tot, fl, nfl = s.last_timing()
+    assert nfl == 4 and 0.0 < fl <= tot
+    assert np.array_equal(s.last_diffs(), want)
+    s.set_flux_timing(False)
+    d = s.step_diffs(1)
+    want = np.array([o.step()])
+    assert np.array_equal(d, want)
+    got, ref = s.download(), o.get_state()
+    assert np.isfinite(ref[0]).all()
+    assert got[3] == ref[3] == done + 2 and got[2] == ref[2]
+    assert_same("cu at the end", got[0], ref[0])
+    s.close()
+
+
+@pytest.mark.parametrize("flux", [2, 1])
 def test_first_order_steps_bitwise(flux):
     """myorder = 1: no reconstruction; the interphase source still comes
     from its own kernel (no gradients or limiter are computed)."""
