@@ -10,11 +10,13 @@ combination of four donor cells of the other mesh,
 summed in that order, after every RK stage (libhydra_hip.so, hp_set_donors).
 The donors are four active cells of the other mesh whose centroids span a
 tetrahedron containing the receptor's centroid, and w are the receptor's
-barycentric coordinates in it: the weights sum to one and reproduce any
-linear field exactly.  When no candidate tetrahedron contains the point,
-the one whose smallest weight is largest is used (a mild extrapolation,
-still exact for linear fields).  A receptor whose centroid coincides with a
-candidate's takes that cell alone (weights 1, 0, 0, 0).
+barycentric coordinates in it: the weights are non-negative, sum to one and
+reproduce any linear field exactly.  The few receptors whose centroid lies
+outside the hull of the other mesh's donor centroids (in its first or last
+cell layer along a boundary it shares) have no containing tetrahedron;
+they take the best one's weights with the negative ones dropped, a convex
+combination.  A receptor whose centroid coincides with a candidate's takes
+that cell alone (weights 1, 0, 0, 0).
 
 File format (ours; the reference has none): ``donor_proc%04d.in``, first
 line the receptor count, then one line per receptor
@@ -50,10 +52,13 @@ def barycentric(p, a, b, c, d):
     return w, det
 
 
-def find_donors(points: np.ndarray, cand_xyz: np.ndarray, cand_ids: np.ndarray, k: int = 8):
-    """For each point, four donors among the k nearest candidate centroids
-    (COMBOS of them) and their barycentric weights; see the module text."""
+def _best_tets(points, cand_xyz, cand_ids, k):
+    """For each point, among all 4-subsets of its k nearest candidate
+    centroids, the non-degenerate one whose smallest barycentric weight is
+    largest: (donor ids, weights, that smallest weight, nearest distance,
+    local cell scale)."""
     from scipy.spatial import cKDTree
+    import itertools
     tree = cKDTree(cand_xyz)
     dist, nn = tree.query(points, k=k)
     n = points.shape[0]
@@ -61,7 +66,8 @@ def find_donors(points: np.ndarray, cand_xyz: np.ndarray, cand_ids: np.ndarray, 
     best_d = np.zeros((n, 4), dtype=np.int64)
     best_min = np.full(n, -np.inf)
     scale = np.linalg.norm(cand_xyz[nn[:, 1]] - cand_xyz[nn[:, 0]], axis=1) ** 3
-    for combo in COMBOS:
+    combos = COMBOS if k == 8 else np.array(list(itertools.combinations(range(k), 4)), dtype=np.int64)
+    for combo in combos:
         idx = nn[:, combo]  # (n, 4)
         w, det = barycentric(points, *(cand_xyz[idx[:, q]] for q in range(4)))
         # reject flat tets: |det| (6 x volume) well below the local cell scale
@@ -71,14 +77,42 @@ def find_donors(points: np.ndarray, cand_xyz: np.ndarray, cand_ids: np.ndarray, 
         best_min = np.where(better, mn, best_min)
         best_w[better] = w[better]
         best_d[better] = cand_ids[idx[better]]
+    return best_d, best_w, best_min, dist[:, 0], scale, nn
+
+
+def find_donors(points: np.ndarray, cand_xyz: np.ndarray, cand_ids: np.ndarray, k: int = 8,
+                widen=(16, 24), tol: float = 1e-12):
+    """For each point, four donors whose centroids span a tetrahedron that
+    contains it, and its barycentric weights (all >= -tol).  The search
+    takes the best tetrahedron over the k nearest candidate centroids, then
+    over the `widen` nearest for the points still outside theirs.  A point
+    outside every candidate tetrahedron (beyond the hull of the candidate
+    centroids, e.g. in a boundary layer of the other mesh) takes the best
+    one's weights with the negative ones set to zero and the rest scaled to
+    sum to one: a convex combination, bounded by its donors' values though
+    no longer exact for linear fields.  Returns (donors, weights, the
+    smallest weight before that clipping)."""
+    best_d, best_w, best_min, d0, scale, nn = _best_tets(points, cand_xyz, cand_ids, k)
+    for kk in widen:
+        out = np.nonzero(best_min < -tol)[0]
+        if out.size == 0 or kk > cand_xyz.shape[0]:
+            break
+        d2, w2, m2, _, _, _ = _best_tets(points[out], cand_xyz, cand_ids, kk)
+        better = m2 > best_min[out]
+        sel = out[better]
+        best_d[sel], best_w[sel], best_min[sel] = d2[better], w2[better], m2[better]
     # a receptor on a candidate centroid takes that cell alone
-    on = dist[:, 0] <= 1e-12 * np.cbrt(scale)
+    on = d0 <= 1e-12 * np.cbrt(scale)
     if on.any():
         best_d[on] = cand_ids[nn[on, :4]]
         best_w[on] = np.array([1.0, 0.0, 0.0, 0.0])
         best_min[on] = 0.0
     if not np.isfinite(best_min).all():
         raise ValueError("donor search: some receptors have no non-degenerate donor tetrahedron")
+    clip = best_min < -tol
+    if clip.any():
+        w = np.maximum(best_w[clip], 0.0)
+        best_w[clip] = w / w.sum(axis=1, keepdims=True)
     return best_d, best_w, best_min
 
 

@@ -41,14 +41,18 @@ def test_donor_weights_partition_of_unity_and_linear(small):
     c = OV.centroids(m)
     assert recv.size > 0 and np.all(m.iblank[recv] == 0) and np.all(m.iblank[donors] == 1)
     assert np.abs(w.sum(axis=1) - 1.0).max() < 1e-13
+    # no extrapolation: every weight non-negative (to roundoff)
+    assert (w >= -1e-12).all()
     rng = np.random.default_rng(4)
     for _ in range(3):
         a, b = rng.normal(), rng.normal(size=3)
         f = a + c @ b
         interp = (w * f[donors]).sum(axis=1)
-        assert np.abs(interp - f[recv]).max() < 1e-12 * (abs(a) + np.abs(b).sum())
-    # most receptors lie inside their donor tetrahedron
-    assert (w.min(axis=1) >= -1e-12).mean() > 0.9
+        exact = np.abs(interp - f[recv]) < 1e-12 * (abs(a) + np.abs(b).sum())
+        # exact for linear fields except beyond the hull of the other mesh's
+        # donor centroids: the first and last cell layers in z
+        assert exact.mean() > 0.9
+        assert (np.abs(c[recv[~exact], 2]) > 0.5 - 1.0 / 16).all()
     # receptors of one mesh take donors from the other (annulus cells,
     # 5 x 24 x 16 hexes x 6, come after the background cells)
     nbg = m.neles - 5 * 24 * 16 * 6
@@ -90,10 +94,21 @@ def test_oracle_gather_reproduces_linear_conserved_fields(small):
         a, b = rng.normal(), rng.normal(size=3)
         cn[k, : m.neles] = a + c @ b
         lin.append((a, b))
+    before = cn[:, : m.neles].copy()
     o.set_donors(recv, donors, w)
     o.call("donor_gather")
+    # receptors whose donor weights reproduce their centroid (all but those
+    # beyond the hull of the other mesh's donor centroids, whose weights
+    # were clipped to a convex combination)
+    inside = np.abs((w[:, :, None] * c[donors]).sum(axis=1) - c[recv]).max(axis=1) < 1e-12
+    assert inside.mean() > 0.9
     for k, (a, b) in enumerate(lin):
-        assert np.abs(cn[k, recv] - (a + c[recv] @ b)).max() < 1e-12 * (abs(a) + np.abs(b).sum())
+        err = np.abs(cn[k, recv] - (a + c[recv] @ b))
+        assert err[inside].max() < 1e-12 * (abs(a) + np.abs(b).sum())
+        f = before[k]
+        want = w[:, 0] * f[donors[:, 0]] + w[:, 1] * f[donors[:, 1]] + w[:, 2] * f[donors[:, 2]] + \
+            w[:, 3] * f[donors[:, 3]]
+        assert np.array_equal(cn[k, recv], want)
 
 
 def test_slab_numbering_keeps_the_case():
@@ -122,3 +137,25 @@ def test_slab_numbering_keeps_the_case():
         d = np.concatenate([np.abs(nc[f] - i)[nc[f] < m.neles] for f in range(4)])
         return np.percentile(d, 90)
     assert spread(m1) < spread(m0)
+
+
+def test_bench_case_donors_do_not_extrapolate():
+    """The two-mesh bench workload (96x96x90 box cut at r = 0.15, annulus
+    10 x 112 x 90 around the r = 0.1 wall, two fringe layers): every one of
+    its 224,877 receptors has weights >= -1e-12 summing to one, so no
+    receptor value leaves the range of its donors'; all but the few whose
+    centroid lies beyond the hull of the other mesh's donor centroids (the
+    first and last cell layers in z) interpolate linear fields exactly."""
+    m, recv, donors, w = OV.overset_mesh(96, 96, 90, nr=10, ntheta=112, r_body=0.1, r_outer=0.25,
+                                         r_cut=0.15, fringe=2, jitter=0.1, seed=1, slabs=(8, 8))
+    assert recv.size == 224877
+    assert (w >= -1e-12).all(), w.min()
+    np.testing.assert_allclose(w.sum(axis=1), 1.0, rtol=0, atol=1e-14)
+    c = OV.centroids(m)
+    g = np.array([0.7, -1.3, 2.1])
+    f = c @ g
+    err = np.abs((w * f[donors]).sum(axis=1) - f[recv])
+    exact = err <= 1e-12
+    assert exact.mean() > 0.995, exact.mean()
+    inexact = c[recv[~exact]]
+    assert (np.abs(inexact[:, 2]) > 0.5 - 1.0 / 90).all()  # the end layers in z
