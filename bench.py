@@ -26,6 +26,7 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
        python bench.py summarize-trace results.db [KERNEL-FOR-A-PER-GRID-SPLIT]
        python bench.py summarize-instructions counter_collection.csv
        python bench.py summarize-pmc counter_collection.csv
+       python bench.py summarize-gaps results.db [NAME-PATTERN]
 
 roofline.traffic is the HBM bytes of one k_flux launch measured with two
 rocprofv3 --pmc runs, of the L2's read requests by size
@@ -276,6 +277,25 @@ def summarize_trace(db_path, top=20, by_grid=None):
             print(f"| {g} | {n} | {d / 1e3:.1f} |")
 
 
+def summarize_gaps(db_path, pattern="%<0,%", window_us=100.0):
+    """Idle time between consecutive kernels of the batched launches (names
+    matching pattern: MODE 0 tile kernels) in a rocprofv3 kernel trace:
+    the gap from one kernel's end to the next one's start, counted when the
+    next starts within window_us (the same graph replay or eager burst).
+    This bounds what fusing a stage's launches into one persistent launch
+    could save.  Prints markdown."""
+    import sqlite3
+    cur = sqlite3.connect(db_path).cursor()
+    rows = sorted(cur.execute("select start, end from kernels where name like ?", (pattern,)))
+    busy = sum(e - s for s, e in rows) / 1e3
+    gaps = [(b[0] - a[1]) / 1e3 for a, b in zip(rows, rows[1:]) if 0 <= (b[0] - a[1]) / 1e3 <= window_us]
+    gaps_arr = np.array(gaps) if gaps else np.zeros(1)
+    print(f"kernels matching `{pattern}`: {len(rows)} launches, {busy / 1e3:.1f} ms busy; "
+          f"{len(gaps)} back-to-back gaps (<= {window_us:.0f} us) summing to {gaps_arr.sum() / 1e3:.2f} ms "
+          f"({100 * gaps_arr.sum() / max(busy, 1e-9):.2f}% of the busy time), median gap "
+          f"{np.median(gaps_arr):.2f} us, 90th percentile {np.percentile(gaps_arr, 90):.2f} us")
+
+
 def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
     """The 64-case uneven ensemble in its FP32-bulk batched form (autotuned
     launch shape, graph-captured step) against the plain one-case FP64 path
@@ -366,6 +386,9 @@ def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-counters":
         summarize_counters(sys.argv[2], sys.argv[3], int(sys.argv[4]))
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "summarize-gaps":
+        summarize_gaps(sys.argv[2], *(sys.argv[3:4]))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-pmc":
         summarize_pmc(sys.argv[2])

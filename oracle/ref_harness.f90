@@ -4,7 +4,8 @@
 ! infrastructure only: used in the build container to pin the C oracle.
 !
 ! usage: ref_harness <input.bin> <output.bin>
-! The input layout is written by tests/golden/make_golden.py; kernel ids:
+! The input layout is written by tests/golden/make_golden.py (and read by
+! oracle/time_ref.py); kernel ids:
 !   1 primitive  2 primitiveplusbc  3 smooth  4 viscousgp
 !   5 venkatakrishnan  6 source  7 hllcbatten  8 hllczien
 !   9 hdr(8) whole time steps composed from the reference's own routines
@@ -27,8 +28,9 @@ program ref_harness
   external primitive, primitiveplusbc, smooth, viscousgp, venkatakrishnan, source
   external hllcbatten, hllczien
   character(len=512) :: fin, fout
-  integer :: hdr(8), kernel, i, j, u, istep
-  real(dp) :: prm(44), qgi, qpi, dtbyvol
+  integer :: hdr(8), kernel, i, j, u, istep, irep, nrep
+  integer(8) :: c0, c1, crate
+  real(dp) :: prm(44), qgi, qpi, dtbyvol, elapsed
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -115,6 +117,12 @@ program ref_harness
     end select
   end do
 
+  ! kernels 1-8 run hdr(8) times (at least once) for timing; the elapsed
+  ! wall time of the calls is written after the state
+  nrep = max(1, hdr(8))
+  if (kernel == 9) nrep = 1
+  call system_clock(c0, crate)
+  do irep = 1, nrep
   select case (kernel)
   case (1); call primitive
   case (2); call primitiveplusbc
@@ -198,6 +206,9 @@ program ref_harness
   case default
     stop 'unknown kernel id'
   end select
+  end do
+  call system_clock(c1)
+  elapsed = real(c1 - c0, dp) / real(crate, dp)
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(u) cn, cu, cn1, d2q, rhs, d24, anu
@@ -209,6 +220,6 @@ program ref_harness
            delphigx, delphigy, delphigz
   write(u) phi
   write(u) visug, visvg, viswg, vistg, visup, visvp, viswp, vistp
-  write(u) dt, time, real(total, dp), diffu, diffv, diffw, diffp
+  write(u) dt, time, real(total, dp), diffu, diffv, diffw, diffp, elapsed
   close(u)
 end program ref_harness

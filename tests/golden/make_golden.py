@@ -172,11 +172,12 @@ def read_output(path, m):
     take("cn", (10, nt)); take("cu", (10, nt)); take("cn1", (10, nt)); take("d2q", (10, nt))
     take("rhs", (10, ne)); take("d24", (10, ne)); take("anu", (3, nt))
     take("q", (18, nt)); take("grad", (39, nt)); take("phi", (nt,)); take("vis", (8, ne))
-    take("trailer", (7,))
+    take("trailer", (8,))
     assert off == raw.size, (off, raw.size)
     t = out.pop("trailer")
     out["dt"], out["time"], out["total"] = t[0], t[1], np.int64(t[2])
     out["diffs"] = t[3:7].copy()  # diffu, diffv, diffw, diffp
+    out["elapsed"] = float(t[7])  # wall seconds of the routine calls
     out["anu"] = out["anu"][:2]
     return out
 
