@@ -369,7 +369,7 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     ALLOC(s.anu, 2 * (size_t)ne) ALLOC(s.rhs, 10 * (size_t)ne) ALLOC(s.scal, SC_N) ALLOC(s.itot, 1) ALLOC(s.mon, 4)
     ALLOC(c->health, 1)
     if (params->precision == 1) {
-        ALLOC(s.g32, (size_t)3 * G_N * ne) ALLOC(s.phi32, ne)
+        if (params->myorder == 2) ALLOC(s.rec32, (size_t)R32_N * ntot)
     } else {
         ALLOC(s.src, (size_t)SRC_N * ne)
     }
@@ -793,15 +793,19 @@ int hp_get_array(hp_ctx *c, int which, double *host, size_t count)
         return HP_OK;
     }
     if (which == 13 || which == 14) {
-        // the FP32-bulk form's gradients and limiter, widened to double
-        const size_t n = which == 13 ? (size_t)3 * G_N * c->m.ne : (size_t)c->m.ne;
-        const float *src = which == 13 ? c->s.g32 : c->s.phi32;
-        if (!src) return fail(c, HP_ERR_ARG, "hp_get_array: arrays 13/14 exist in the FP32-bulk form only");
+        // the FP32-bulk form's gradients and limiter, from the cells' records
+        const size_t ne = c->m.ne, n = which == 13 ? (size_t)3 * G_N * ne : ne;
+        if (!c->s.rec32) return fail(c, HP_ERR_ARG, "hp_get_array: arrays 13/14 exist in the FP32-bulk second-order form only");
         if (count != n) return fail(c, HP_ERR_ARG, "hp_get_array: count mismatch");
-        std::vector<float> tmp(n);
-        HIPCHK(c, hipMemcpyAsync(tmp.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, c->st));
+        std::vector<float> tmp((size_t)R32_N * ne);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), c->s.rec32, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
-        for (size_t k = 0; k < n; k++) host[k] = tmp[k];
+        for (size_t i = 0; i < ne; i++) {
+            const float *r = &tmp[(size_t)R32_N * i];
+            if (which == 14) host[i] = r[R32_PHI];
+            else
+                for (int k = 0; k < 3 * G_N; k++) host[(size_t)k * ne + i] = r[R32_GRAD + k];
+        }
         return HP_OK;
     }
     double *p;
@@ -1184,7 +1188,7 @@ size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision)
     b += (13 * ne + 4 * ng) * I;                             // nod, nc, ncp, iblank, ghost lists
     b += (12 + 3 + 12 + 12) * ne * D + 4 * nt * D;         // sc, dl/epssq/dd, fc0, fc3, xc..vol
     b += (30 + 3 * G_N + 1 + 8 + 10 + 10 + 2 + 10) * ne * D + 2 * (size_t)Q_N * nt * D + (SC_N + 4) * D + I;
-    if (precision == 1) b += (3 * G_N + 1) * ne * sizeof(float);  // FP32 gradients, limiter
+    if (precision == 1) b += (size_t)R32_N * nt * sizeof(float);  // FP32-bulk records
     else b += SRC_N * ne * D;                                     // interphase source terms
     const size_t tc = (ne + TILE - 1) / TILE, tg = (ng + TILE - 1) / TILE + 1;
     b += sizeof(Params) + sizeof(State) + sizeof(Mesh) + (tc + tg) * sizeof(Tile) + 4 * I + 4 * tc * D;

@@ -23,6 +23,13 @@ enum { SRC_MX, SRC_MY, SRC_MZ, SRC_E, SRC_MXP, SRC_N };
 enum { G_UG, G_VG, G_WG, G_TG, G_UP, G_VP, G_WP, G_TP, G_PG, G_PP, G_PHIG, G_N };
 // device scalars
 enum { SC_DT, SC_TIME, SC_DTMIN, SC_N };
+// The FP32-bulk record of a cell (or ghost), 48 floats = 192 bytes read as
+// twelve 16-byte loads: the face state's 13 primitives (rog ug vg wg tg phig
+// rop up vp wp tp pg pp, rounded from FP64), the 11 stored gradients x
+// (x, y, z) in G_* order, the limiter, one pad.  A ghost's record holds its
+// primitives only; a cell that is never updated keeps zero gradients and
+// limiter, as the reference keeps its derivative arrays.
+enum { R32_PRIM = 0, R32_GRAD = 13, R32_PHI = 46, R32_N = 48 };
 
 // CFL bound of one cell from its primitives (main.F90:525-546): dl over the
 // larger phase sound speed plus the larger phase speed.  q is variable-major
@@ -66,7 +73,8 @@ struct State {
     double *grad;           // 3*G_N x ne
     double *phi;            // ne
     double *src;            // SRC_N x ne interphase source terms x volume (FP64 form; else null)
-    float *g32, *phi32;     // FP32-bulk variant: gradients and limiter (else null)
+    float *rec32;           // FP32-bulk variant with second-order faces: per cell and ghost (ntot rows,
+                            // cell-major) R32_N floats, the record the face loop reads (else null)
     double *vis;            // 8 x ne
     double *d24, *d2q;      // 10 x ne
     double *anu;            // 2 x ne

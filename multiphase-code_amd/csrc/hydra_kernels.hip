@@ -1409,27 +1409,49 @@ __device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float det3f(float a11, float a21, float a31, float a12, float a22, float a32,
                                        float a13, float a23, float a33)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     return a11 * (a22 * a33 - a23 * a32) - a21 * (a12 * a33 - a13 * a32) + a31 * (a12 * a23 - a13 * a22);
 }
 
 __device__ __forceinline__ float fvenkatf(float delp, float delm, float epssq)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const float num = (delp * delp + epssq) * delm + 2.0f * delm * delm * delp;
     const float den = delp * delp + 2.0f * delm * delm + delp * delm + epssq;
     return num * frcp(den * delm);
 }
 
 // gradients (sr_derivative.F90:45-322, incl. the cTp2 defect) and limiter
-// (sr_VK.F90:32-274) in FP32, one thread per cell
-__device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int i)
+// (sr_VK.F90:32-274) in FP32, one thread per cell, written with the cell's
+// primitives as its record (R32_*)
+__device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int i, float *lrec)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const int ne = m.ne;
-    if (m.iblank[i] != 1) return;
     const double *q = s.q;
     const size_t nt = s.nt;
+    // the cell's record: its primitives whatever its iblank; gradients and
+    // limiter of an updated cell (a frozen one keeps zeros)
+    float r[R32_N];
+    r[0] = (float)QV(Q_ROG, i); r[1] = (float)QV(Q_UG, i); r[2] = (float)QV(Q_VG, i);
+    r[3] = (float)QV(Q_WG, i); r[4] = (float)QV(Q_TG, i); r[5] = (float)QV(Q_PHIG, i);
+    r[6] = (float)QV(Q_ROP, i); r[7] = (float)QV(Q_UP, i); r[8] = (float)QV(Q_VP, i);
+    r[9] = (float)QV(Q_WP, i); r[10] = (float)QV(Q_TP, i); r[11] = (float)QV(Q_PG, i);
+    r[12] = (float)QV(Q_PP, i);
+#pragma unroll
+    for (int k = R32_GRAD; k < R32_N; k++) r[k] = 0.0f;
+    // the record goes to the wave's LDS staging (lrec: this lane's R32_N
+    // floats) a 16-byte group at a time, as soon as the group's four values
+    // are known; the wave then writes the tile's records out contiguously
+    float4 *dst = reinterpret_cast<float4 *>(lrec);
+    auto put = [&](int j) { dst[j] = make_float4(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]); };
+#pragma unroll
+    for (int j = 0; j < 3; j++) put(j);
+    if (m.iblank[i] != 1) {
+#pragma unroll
+        for (int j = 3; j < R32_N / 4; j++) put(j);
+        return;
+    }
     int nb[4];
     for (int k = 0; k < 4; k++) nb[k] = m.nc[(size_t)k * ne + i];
     const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i];
@@ -1467,9 +1489,14 @@ __device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int
         const float gy = det3f(a1, b1, b2, c1, c2, c3, b2, b3, a3) * idn;
         const float gz = det3f(a1, b1, b2, b1, a2, b3, c1, c2, c3) * idn;
         if (sI < 11) {
-            s.g32[(size_t)(3 * sI + 0) * ne + i] = gx;
-            s.g32[(size_t)(3 * sI + 1) * ne + i] = gy;
-            s.g32[(size_t)(3 * sI + 2) * ne + i] = gz;
+            r[R32_GRAD + 3 * sI + 0] = gx;
+            r[R32_GRAD + 3 * sI + 1] = gy;
+            r[R32_GRAD + 3 * sI + 2] = gz;
+            // groups whose last value is this slot's (index R32_GRAD + 3 sI + 2)
+            // or an earlier one, and not complete before it
+#pragma unroll
+            for (int j = 3; j < R32_N / 4 - 1; j++)
+                if (4 * j + 3 <= R32_GRAD + 3 * sI + 2 && 4 * j + 3 > R32_GRAD + 3 * sI - 1) put(j);
         }
         const int li = (sl == Q_PG) ? 0 : (sl == Q_PP) ? 1 : (sl == Q_ROG) ? 2 : (sl == Q_ROP) ? 3 : -1;
         if (li >= 0) { glim[li][0] = gx; glim[li][1] = gy; glim[li][2] = gz; }
@@ -1498,7 +1525,8 @@ __device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int
             ph = (v == 0 && f == 0) ? pv : fmn(ph, pv);
         }
     }
-    s.phi32[i] = ph;
+    r[R32_PHI] = ph;
+    put(R32_N / 4 - 1);
 }
 
 struct FState32 {
@@ -1517,7 +1545,7 @@ __device__ __forceinline__ void load_fstate32(const double *q, size_t nt, int i,
 // sr_hllcbatten.F90:228-318 in FP32
 __device__ __forceinline__ void clip_blend32(const Params &P, FState32 &s)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const float emin = (float)P.epslnmin, emax = (float)P.epslnmax, ide = frcp(emax - emin);
     if (s.phig < emin) {
         s.phig = emin; s.phip = 1.0f - s.phig;
@@ -1556,7 +1584,7 @@ __device__ __forceinline__ void hllc_phase32(const Params &P, float phL, float r
                                              float tR, float pR, float enx, float eny, float enz, float area,
                                              float *F)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const float irL = frcp(roL), irR = frcp(roR);
     float EL, ER, aL, aR;
     if (GAS) {
@@ -1627,7 +1655,7 @@ __device__ __forceinline__ void hllc_phase32(const Params &P, float phL, float r
 __device__ __forceinline__ void hllc_zien32(const Params &P, const FState32 &L, const FState32 &R, float enx,
                                             float eny, float enz, float area, float *F)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const float cvg = (float)(P.rrg / (P.gamag - 1.0)), gg = (float)P.gamag;
     const float cvp = (float)(P.cpp / P.gamap), gp = (float)P.gamap, pinf = (float)P.pinf;
     const float irgL = frcp(L.rog), irgR = frcp(R.rog), irpL = frcp(L.rop), irpR = frcp(R.rop);
@@ -1713,7 +1741,7 @@ __device__ __forceinline__ void source_and_update_f32(const Params &P, const Sta
                                                       int f, int qbase, int neigh, const double *rhs, int stage,
                                                       bool store, double (&cv)[3])
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const int ne = m.ne;
     const double *q = s.q;
     const size_t nt = s.nt;
@@ -1783,7 +1811,12 @@ __device__ __forceinline__ void source_and_update_f32(const Params &P, const Sta
     }
 }
 
-#define G32(sl, d, c) s.g32[(size_t)(3 * (sl) + (d)) * ne + (c)]
+// a face state from a record's primitives (R32_PRIM order)
+__device__ __forceinline__ void rec_state32(const float (&r)[R32_N], FState32 &st)
+{
+    st.rog = r[0]; st.ug = r[1]; st.vg = r[2]; st.wg = r[3]; st.tg = r[4]; st.phig = r[5]; st.rop = r[6];
+    st.up = r[7]; st.vp = r[8]; st.wp = r[9]; st.tp = r[10]; st.pg = r[11]; st.pp = r[12];
+}
 
 // face loop in FP32, one cell per quad of lanes as in k_flux; the quad's
 // face fluxes are widened to FP64 and summed in face order, then the FP64
@@ -1792,7 +1825,7 @@ template <int FLUX>
 __device__ __forceinline__ void flux_update_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
                                                 int qbase, int stage, bool store, double (&cv)[3])
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const int ne = m.ne;
     const double *q = s.q;
     const size_t nt = s.nt;
@@ -1805,21 +1838,39 @@ __device__ __forceinline__ void flux_update_f32(const Params &P, const State &s,
         const float enx = scx * ia, eny = scy * ia, enz = scz * ia;
         const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
         FState32 L, R;
-        load_fstate32(q, nt, i, L);
-        load_fstate32(q, nt, neigh, R);
-        if (P.myorder == 2 && neigh < ne) {
-            const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
-                         zf = m.fc0[(size_t)(3 * f + 2) * ne + i];
-            const float phimin = fmn(s.phi32[i], s.phi32[neigh]);
-            const float xf0 = (float)(xf - m.xc[i]), yf0 = (float)(yf - m.yc[i]), zf0 = (float)(zf - m.zc[i]);
-            const float xfn = (float)(xf - m.xc[neigh]), yfn = (float)(yf - m.yc[neigh]),
-                        zfn = (float)(zf - m.zc[neigh]);
-#define REC32(fld, sl)                                                                          \
-    L.fld += phimin * (xf0 * G32(sl, 0, i) + yf0 * G32(sl, 1, i) + zf0 * G32(sl, 2, i));       \
-    R.fld += phimin * (xfn * G32(sl, 0, neigh) + yfn * G32(sl, 1, neigh) + zfn * G32(sl, 2, neigh));
-            REC32(phig, G_PHIG) REC32(ug, G_UG) REC32(vg, G_VG) REC32(wg, G_WG) REC32(tg, G_TG)
-            REC32(pg, G_PG) REC32(up, G_UP) REC32(vp, G_VP) REC32(wp, G_WP) REC32(tp, G_TP) REC32(pp, G_PP)
+        if (P.myorder == 2) {
+            // both sides' records (cell i's and its neighbour's, a ghost's
+            // holding its primitives), twelve 16-byte loads each
+            float a[R32_N], b[R32_N];
+            const float4 *ra = reinterpret_cast<const float4 *>(s.rec32) + (size_t)(R32_N / 4) * i;
+            const float4 *rb = reinterpret_cast<const float4 *>(s.rec32) + (size_t)(R32_N / 4) * neigh;
+#pragma unroll
+            for (int j = 0; j < R32_N / 4; j++) {
+                const float4 u = ra[j], v = rb[j];
+                a[4 * j] = u.x; a[4 * j + 1] = u.y; a[4 * j + 2] = u.z; a[4 * j + 3] = u.w;
+                b[4 * j] = v.x; b[4 * j + 1] = v.y; b[4 * j + 2] = v.z; b[4 * j + 3] = v.w;
+            }
+            rec_state32(a, L);
+            rec_state32(b, R);
+            if (neigh < ne) {
+                const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
+                             zf = m.fc0[(size_t)(3 * f + 2) * ne + i];
+                const float phimin = fmn(a[R32_PHI], b[R32_PHI]);
+                const float xf0 = (float)(xf - m.xc[i]), yf0 = (float)(yf - m.yc[i]), zf0 = (float)(zf - m.zc[i]);
+                const float xfn = (float)(xf - m.xc[neigh]), yfn = (float)(yf - m.yc[neigh]),
+                            zfn = (float)(zf - m.zc[neigh]);
+#define REC32(fld, sl)                                                                                   \
+    L.fld += phimin * (xf0 * a[R32_GRAD + 3 * (sl)] + yf0 * a[R32_GRAD + 3 * (sl) + 1] +                 \
+                       zf0 * a[R32_GRAD + 3 * (sl) + 2]);                                                \
+    R.fld += phimin * (xfn * b[R32_GRAD + 3 * (sl)] + yfn * b[R32_GRAD + 3 * (sl) + 1] +                 \
+                       zfn * b[R32_GRAD + 3 * (sl) + 2]);
+                REC32(phig, G_PHIG) REC32(ug, G_UG) REC32(vg, G_VG) REC32(wg, G_WG) REC32(tg, G_TG)
+                REC32(pg, G_PG) REC32(up, G_UP) REC32(vp, G_VP) REC32(wp, G_WP) REC32(tp, G_TP) REC32(pp, G_PP)
 #undef REC32
+            }
+        } else {
+            load_fstate32(q, nt, i, L);
+            load_fstate32(q, nt, neigh, R);
         }
         L.phip = 1.0f - L.phig;
         R.phip = 1.0f - R.phig;
@@ -1878,7 +1929,7 @@ __device__ __forceinline__ void viscous_stencil_f32(const Params &P, const doubl
                                                     const double (&fv)[8], float phigf, float phipf,
                                                     float (&acc)[8])
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const int slot[8] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP};
     const float tgf = (float)fv[3];
     const float amug = (float)P.as * fsqr(tgf) * frcp(1.0f + (float)P.ts * frcp(tgf));
@@ -2081,6 +2132,15 @@ __global__ void __launch_bounds__(TILE) k_bc(Batch B, int viscous)
             // the last stage's ghosts.
             const size_t nt = C.S.nt, o = (size_t)C.S.ne + g;
             for (int k = 0; k < Q_N; k++) C.S.q2[(size_t)k * nt + o] = C.S.q[(size_t)k * nt + o];
+            if (C.S.rec32) {
+                // the ghost's FP32-bulk record: its primitives (R32_PRIM order)
+                const double *q = C.S.q;
+                float4 *dst = reinterpret_cast<float4 *>(C.S.rec32) + (size_t)(R32_N / 4) * o;
+                dst[0] = make_float4((float)QV(Q_ROG, o), (float)QV(Q_UG, o), (float)QV(Q_VG, o), (float)QV(Q_WG, o));
+                dst[1] = make_float4((float)QV(Q_TG, o), (float)QV(Q_PHIG, o), (float)QV(Q_ROP, o), (float)QV(Q_UP, o));
+                dst[2] = make_float4((float)QV(Q_VP, o), (float)QV(Q_WP, o), (float)QV(Q_TP, o), (float)QV(Q_PG, o));
+                dst[3] = make_float4((float)QV(Q_PP, o), 0.0f, 0.0f, 0.0f);
+            }
         }
     }
 }
@@ -2117,10 +2177,18 @@ __global__ void __launch_bounds__((!F32 && GRAD) ? 2 * TILE : TILE) k_grad_limit
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
         const int lane = (int)(threadIdx.x & 63u);
-        if (lane < C.count) {
-            if (F32)
-                grad_cell_f32(C.S, C.M, C.first + lane);
-            else if (GRAD && threadIdx.x < TILE)
+        if (F32) {
+            // records staged in LDS, then stored as one contiguous run of
+            // the tile's count x R32_N floats (consecutive cells' records
+            // are adjacent)
+            __shared__ float4 stage32[TILE * R32_N / 4];
+            if (lane < C.count) grad_cell_f32(C.S, C.M, C.first + lane, reinterpret_cast<float *>(stage32) + R32_N * lane);
+            __syncthreads();
+            float4 *out = reinterpret_cast<float4 *>(C.S.rec32) + (size_t)(R32_N / 4) * C.first;
+            for (int k = lane; k < C.count * (R32_N / 4); k += TILE) out[k] = stage32[k];
+            if (MODE == 1) __syncthreads();  // the next tile reuses the staging
+        } else if (lane < C.count) {
+            if (GRAD && threadIdx.x < TILE)
                 grad_cell(C.S, C.M, C.first + lane);
             else
                 source_cell(C.P, C.S, C.M, C.first + lane);
