@@ -2086,7 +2086,17 @@ __global__ void __launch_bounds__(1024) k_dt_final(Batch B)
     const int t0 = B.slot_t0[c], t1 = B.slot_t1[c];
     if (t0 >= t1) return;
     double v = 1.0e8;
-    for (int t = t0 + (int)threadIdx.x; t < t1; t += (int)blockDim.x) v = dmin(v, B.part[(size_t)4 * t]);
+    // eight independent loads in flight per thread (the partials of a
+    // 5M-cell case are 78k tiles; one load at a time left this launch
+    // waiting on latency)
+    const int bd = (int)blockDim.x;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += 8 * bd) {
+        double w[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) w[j] = (t + j * bd < t1) ? B.part[(size_t)4 * (t + j * bd)] : 1.0e8;
+#pragma unroll
+        for (int j = 0; j < 8; j++) v = dmin(v, w[j]);
+    }
     __shared__ double red[16];
     v = wave_min(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -2482,8 +2492,19 @@ __global__ void __launch_bounds__(1024) k_monitor_final(Batch B, const double *d
     const int t0 = B.slot_t0[c], t1 = B.slot_t1[c];
     if (t0 >= t1) return;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int t = t0 + (int)threadIdx.x; t < t1; t += (int)blockDim.x)
-        for (int k = 0; k < 4; k++) v[k] = dmax(v[k], B.part[(size_t)4 * t + k]);
+    const int bd = (int)blockDim.x;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += 4 * bd) {
+        double4 w[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            w[j] = (t + j * bd < t1) ? reinterpret_cast<const double4 *>(B.part)[t + j * bd]
+                                     : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            v[0] = dmax(v[0], w[j].x); v[1] = dmax(v[1], w[j].y);
+            v[2] = dmax(v[2], w[j].z); v[3] = dmax(v[3], w[j].w);
+        }
+    }
     for (int b = (int)threadIdx.x; b < ndpart; b += (int)blockDim.x)
         for (int k = 0; k < 4; k++) v[k] = dmax(v[k], dpart[(size_t)4 * b + k]);
     __shared__ double red[4][16];
