@@ -26,7 +26,8 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n NX NY NZ]
        python bench.py summarize-trace results.db [KERNEL-FOR-A-PER-GRID-SPLIT]
        python bench.py summarize-instructions counter_collection.csv
        python bench.py summarize-pmc counter_collection.csv
-       python bench.py summarize-gaps results.db [NAME-PATTERN]
+       python bench.py summarize-gaps results.db [WINDOW-US]
+       python bench.py run-ensemble [graph|eager] [STEPS]
 
 roofline.traffic is the HBM bytes of one k_flux launch measured with two
 rocprofv3 --pmc runs, of the L2's read requests by size
@@ -277,23 +278,36 @@ def summarize_trace(db_path, top=20, by_grid=None):
             print(f"| {g} | {n} | {d / 1e3:.1f} |")
 
 
-def summarize_gaps(db_path, pattern="%<0,%", window_us=100.0):
-    """Idle time between consecutive kernels of the batched launches (names
-    matching pattern: MODE 0 tile kernels) in a rocprofv3 kernel trace:
-    the gap from one kernel's end to the next one's start, counted when the
-    next starts within window_us (the same graph replay or eager burst).
-    This bounds what fusing a stage's launches into one persistent launch
-    could save.  Prints markdown."""
+def summarize_gaps(db_path, window_us=100.0, min_launches=100):
+    """Idle time of the GPU between consecutive kernels of a rocprofv3
+    kernel trace: kernels in start order, the idle time before each one its
+    start minus the latest end so far.  An idle stretch longer than
+    window_us separates bursts (host work between calls); inside the bursts
+    of at least min_launches launches (the stepping), the idle time is what
+    fusing the step's launches into one persistent launch could save at
+    most.  Prints markdown."""
     import sqlite3
     cur = sqlite3.connect(db_path).cursor()
-    rows = sorted(cur.execute("select start, end from kernels where name like ?", (pattern,)))
-    busy = sum(e - s for s, e in rows) / 1e3
-    gaps = [(b[0] - a[1]) / 1e3 for a, b in zip(rows, rows[1:]) if 0 <= (b[0] - a[1]) / 1e3 <= window_us]
-    gaps_arr = np.array(gaps) if gaps else np.zeros(1)
-    print(f"kernels matching `{pattern}`: {len(rows)} launches, {busy / 1e3:.1f} ms busy; "
-          f"{len(gaps)} back-to-back gaps (<= {window_us:.0f} us) summing to {gaps_arr.sum() / 1e3:.2f} ms "
-          f"({100 * gaps_arr.sum() / max(busy, 1e-9):.2f}% of the busy time), median gap "
-          f"{np.median(gaps_arr):.2f} us, 90th percentile {np.percentile(gaps_arr, 90):.2f} us")
+    rows = sorted(cur.execute("select start, end from kernels"))
+    st = np.array([r[0] for r in rows], dtype=np.float64)
+    en = np.maximum.accumulate(np.array([r[1] for r in rows], dtype=np.float64))
+    idle = np.concatenate([[np.inf], (st[1:] - en[:-1]) / 1e3])  # us before each launch
+    cuts = np.flatnonzero(idle > window_us).tolist() + [len(rows)]
+    print("| burst | launches | span us | kernel busy us | idle us | idle % | median gap us |")
+    print("|---|---|---|---|---|---|---|")
+    tot_span = tot_idle = 0.0
+    for a, b in zip(cuts, cuts[1:]):
+        if b - a < min_launches:
+            continue
+        gaps = np.clip(idle[a + 1:b], 0.0, None)
+        span = (en[b - 1] - st[a]) / 1e3
+        busy = span - gaps.sum()
+        tot_span += span
+        tot_idle += gaps.sum()
+        print(f"| {a} | {b - a} | {span:.1f} | {busy:.1f} | {gaps.sum():.1f} | "
+              f"{100 * gaps.sum() / span:.3f} | {np.median(gaps):.2f} |")
+    print(f"\nbursts of >= {min_launches} launches: {tot_idle:.1f} us idle in {tot_span:.1f} us "
+          f"({100 * tot_idle / max(tot_span, 1e-9):.3f}%)")
 
 
 def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
@@ -355,6 +369,36 @@ def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
             "cases_unhealthy_at_end": int(sum(1 for h in res["health"] if h))}
 
 
+def run_ensemble_steps(mode="graph", steps=10, warmup=2):
+    """The 64-case FP32-bulk ensemble of ensemble_bench in one batch with the
+    default launch shape (one block per tile), stepped warmup then `steps`
+    times, by graph replays or eager launches: the run whose kernel trace
+    shows the launch gaps of the batched step.  Prints ms/step."""
+    if mode not in ("graph", "eager"):
+        raise SystemExit(f"run-ensemble: mode {mode!r} is neither graph nor eager")
+    from hydraphase import Solver
+    from hydraphase import batch as HB
+    specs = HB.ensemble_specs(64, seed=1)
+    cases = [HB.build_case(sp, 1) for sp in specs]
+    b = HB.Batch(*HB.capacity([c[0] for c in cases]))
+    sols = []
+    for (m, p, cu, gp, _) in cases:
+        s = Solver(m, p)
+        s.upload(cu, gp, 0.0, 0)
+        b.add(s)
+        sols.append(s)
+    print(f"# {len(cases)} cases built", file=sys.stderr, flush=True)
+    graph = mode == "graph"
+    b.step(warmup, graph=graph)
+    b.synchronize()
+    b.step(steps, graph=graph)
+    b.synchronize()
+    print(json.dumps({"mode": mode, "steps": steps, "ms_per_step": b.last_ms() / steps}), flush=True)
+    b.close()
+    for s in sols:
+        s.close()
+
+
 def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
     """Poisson arrivals of cases (n 10..70, 50..500 steps), admitted smallest
     work first within a 200 GB HBM budget while at most 1M active cells are
@@ -388,13 +432,16 @@ def main():
         summarize_counters(sys.argv[2], sys.argv[3], int(sys.argv[4]))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-gaps":
-        summarize_gaps(sys.argv[2], *(sys.argv[3:4]))
+        summarize_gaps(sys.argv[2], *map(float, sys.argv[3:4]))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-pmc":
         summarize_pmc(sys.argv[2])
         return
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-instructions":
         summarize_instructions(sys.argv[2])
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "run-ensemble":
+        run_ensemble_steps(*(sys.argv[2:3]), *map(int, sys.argv[3:4]))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-trace":
         summarize_trace(sys.argv[2], by_grid=sys.argv[3] if len(sys.argv) > 3 else None)

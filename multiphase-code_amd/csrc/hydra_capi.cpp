@@ -322,6 +322,48 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
         const int b = hnc[k];
         hncp[k] = b < ne ? b : gpar[b - ne];
     }
+    // FP32-bulk HLLC-Batten: the face tables of State::fslot/fbase/flist.  A
+    // cell lists its boundary faces and the faces towards higher-numbered
+    // cells; the other cell of a listed face takes that entry, negated.  A
+    // pair of cells joined by more than one face (not a valid mesh) keeps a
+    // flux per side.
+    std::vector<int> hfslot, hfbase, hflist;
+    if (params->precision == 1 && params->myflux == 2) {
+        if (ne >= (1 << 28)) return fail(nullptr, HP_ERR_ARG, "hp_create: FP32-bulk face tables take < 2^28 cells");
+        hfslot.assign(4 * (size_t)ne, 0);
+        hfbase.assign((size_t)ne + 1, 0);
+        hflist.reserve(3 * (size_t)ne);
+        // the face of j towards i when i and j share exactly one face, else -1
+        auto back = [&](int i, int j) -> int {
+            int nij = 0, nji = 0, fb = -1;
+            for (int k = 0; k < 4; k++) {
+                if (hnc[(size_t)k * ne + i] == j) nij++;
+                if (hnc[(size_t)k * ne + j] == i) {
+                    nji++;
+                    fb = k;
+                }
+            }
+            return (nij == 1 && nji == 1) ? fb : -1;
+        };
+        for (int i = 0; i < ne; i++) {
+            hfbase[i] = (int)hflist.size();
+            for (int f = 0; f < 4; f++) {
+                const int j = hnc[(size_t)f * ne + i];
+                if (j >= ne || i < j || back(i, j) < 0) {
+                    hfslot[(size_t)f * ne + i] = (int)hflist.size() << 1;
+                    hflist.push_back(4 * i + f);
+                }
+            }
+        }
+        hfbase[ne] = (int)hflist.size();
+        for (int i = 0; i < ne; i++)
+            for (int f = 0; f < 4; f++) {
+                const int j = hnc[(size_t)f * ne + i];
+                if (j >= ne || i < j) continue;
+                const int fb = back(i, j);
+                if (fb >= 0) hfslot[(size_t)f * ne + i] = hfslot[(size_t)fb * ne + j] | 1;
+            }
+    }
     double xmax = -1.0e30, ymax = -1.0e30, zmax = -1.0e30, ymin = 1.0e30, zmin = 1.0e30;
     for (int n = 0; n < nodes; n++) {
         xmax = std::max(xmax, x[n]);
@@ -368,8 +410,13 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     ALLOC(s.vis, 8 * (size_t)ne) ALLOC(s.d24, 10 * (size_t)ne) ALLOC(s.d2q, 10 * (size_t)ne)
     ALLOC(s.anu, 2 * (size_t)ne) ALLOC(s.rhs, 10 * (size_t)ne) ALLOC(s.scal, SC_N) ALLOC(s.itot, 1) ALLOC(s.mon, 4)
     ALLOC(c->health, 1)
+    int *dfslot = nullptr, *dfbase = nullptr, *dflist = nullptr;
     if (params->precision == 1) {
         if (params->myorder == 2) ALLOC(s.rec32, (size_t)R32_N * ntot)
+        if (!hflist.empty()) {
+            ALLOC(dfslot, hfslot.size()) ALLOC(dfbase, hfbase.size()) ALLOC(dflist, hflist.size())
+            ALLOC(s.fflux, (size_t)FACE32_N * hflist.size())
+        }
     } else {
         ALLOC(s.src, (size_t)SRC_N * ne)
     }
@@ -377,6 +424,9 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     m.x = dx; m.y = dy; m.z = dz; m.nod = dnod; m.nc = dnc; m.ncp = dncp; m.iblank = dib;
     m.gpar = dgpar; m.gtype = dgtype; m.gside = dgside; m.gdd_win = dgwin;
     s.dl = m.dl;
+    s.fslot = dfslot;
+    s.fbase = dfbase;
+    s.flist = dflist;
     auto h2d = [&](void *d, const void *h, size_t bytes) -> int {
         hipError_t ee = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->st);
         if (ee != hipSuccess) return fail(c, HP_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(ee));
@@ -388,7 +438,10 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
         (rc = h2d(dncp, hncp.data(), sizeof(int) * hncp.size())) ||
         (rc = h2d(dib, hib.data(), sizeof(int) * hib.size())) ||
         (ng && ((rc = h2d(dgpar, gpar.data(), sizeof(int) * ng)) || (rc = h2d(dgtype, gtype.data(), sizeof(int) * ng)) ||
-                (rc = h2d(dgside, gside.data(), sizeof(int) * ng)) || (rc = h2d(dgwin, gwin.data(), sizeof(int) * ng))))) {
+                (rc = h2d(dgside, gside.data(), sizeof(int) * ng)) || (rc = h2d(dgwin, gwin.data(), sizeof(int) * ng)))) ||
+        (dflist && ((rc = h2d(dfslot, hfslot.data(), sizeof(int) * hfslot.size())) ||
+                    (rc = h2d(dfbase, hfbase.data(), sizeof(int) * hfbase.size())) ||
+                    (rc = h2d(dflist, hflist.data(), sizeof(int) * hflist.size()))))) {
         g_create_err = c->err;
         hp_destroy(c);
         return rc;
@@ -950,8 +1003,10 @@ static void batch_grids(hp_batch *b)
                    (b->G.ghost < b->ts.gt_cap ? 4 : 0);
 }
 
+// forget the captured step; a launch of it still in flight finishes first
 static void batch_drop_graph(hp_batch *b)
 {
+    if (b->exec && b->st) (void)hipStreamSynchronize(b->st);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -1188,7 +1243,10 @@ size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision)
     b += (13 * ne + 4 * ng) * I;                             // nod, nc, ncp, iblank, ghost lists
     b += (12 + 3 + 12 + 12) * ne * D + 4 * nt * D;         // sc, dl/epssq/dd, fc0, fc3, xc..vol
     b += (30 + 3 * G_N + 1 + 8 + 10 + 10 + 2 + 10) * ne * D + 2 * (size_t)Q_N * nt * D + (SC_N + 4) * D + I;
-    if (precision == 1) b += (size_t)R32_N * nt * sizeof(float);  // FP32-bulk records
+    if (precision == 1) {
+        b += (size_t)R32_N * nt * sizeof(float);  // FP32-bulk records
+        b += (9 * ne + 1) * I + 4 * ne * FACE32_N * sizeof(float);  // face tables and entries (HLLC-Batten; at most)
+    }
     else b += SRC_N * ne * D;                                     // interphase source terms
     const size_t tc = (ne + TILE - 1) / TILE, tg = (ng + TILE - 1) / TILE + 1;
     b += sizeof(Params) + sizeof(State) + sizeof(Mesh) + (tc + tg) * sizeof(Tile) + 4 * I + 4 * tc * D;

@@ -30,6 +30,8 @@ enum { SC_DT, SC_TIME, SC_DTMIN, SC_N };
 // primitives only; a cell that is never updated keeps zero gradients and
 // limiter, as the reference keeps its derivative arrays.
 enum { R32_PRIM = 0, R32_GRAD = 13, R32_PHI = 46, R32_N = 48 };
+// floats per face entry of State::fflux
+enum { FACE32_N = 16 };
 
 // CFL bound of one cell from its primitives (main.F90:525-546): dl over the
 // larger phase sound speed plus the larger phase speed.  q is variable-major
@@ -75,6 +77,17 @@ struct State {
     double *src;            // SRC_N x ne interphase source terms x volume (FP64 form; else null)
     float *rec32;           // FP32-bulk variant with second-order faces: per cell and ghost (ntot rows,
                             // cell-major) R32_N floats, the record the face loop reads (else null)
+    // FP32-bulk HLLC-Batten (else null): every face's flux computed once for
+    // the two cells that share it.  The faces a cell computes (a boundary
+    // face, or one towards a higher-numbered cell) are listed cell by cell:
+    // flist[k] = 4*cell + face, cell i's from fbase[i] to fbase[i+1] (ne+1
+    // entries); fflux holds FACE32_N floats per listed face (10 flux
+    // components, the phig difference and centroid offset x, y, z towards
+    // the other cell, 2 pad); fslot (4 x ne, face-major) gives each cell
+    // face its entry as (k << 1) | 1 when the cell takes the listed values
+    // with the opposite sign
+    const int *fslot, *fbase, *flist;
+    float *fflux;
     double *vis;            // 8 x ne
     double *d24, *d2q;      // 10 x ne
     double *anu;            // 2 x ne

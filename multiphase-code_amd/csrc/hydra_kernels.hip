@@ -1582,7 +1582,7 @@ template <bool GAS>
 __device__ __forceinline__ void hllc_phase32(const Params &P, float phL, float roL, float uL, float vL, float wL,
                                              float tL, float pL, float phR, float roR, float uR, float vR, float wR,
                                              float tR, float pR, float enx, float eny, float enz, float area,
-                                             float *F)
+                                             float (&Fa)[10])
 {
 #pragma clang fp contract(on)
     const float irL = frcp(roL), irR = frcp(roR);
@@ -1610,12 +1610,14 @@ __device__ __forceinline__ void hllc_phase32(const Params &P, float phL, float r
     const float SS = snr * frcp(sdr + 1.0e-8f);
     const float pst = fmx(pL + roL * (qL - SL) * (qL - SS), 1.0e-6f * fmn(pL, pR));
     const float mL = phL * roL, mR = phR * roR;
+    // the branch's five components, stored once below (gas 0-4, liquid 5-9)
+    float r0, r1, r2, r3, r4;
     if (SL > 0.0f) {
-        F[0] = mL * qL * area;
-        F[1] = (mL * qL * uL + phL * pL * enx) * area;
-        F[2] = (mL * qL * vL + phL * pL * eny) * area;
-        F[3] = (mL * qL * wL + phL * pL * enz) * area;
-        F[4] = mL * qL * HL * area;
+        r0 = mL * qL * area;
+        r1 = (mL * qL * uL + phL * pL * enx) * area;
+        r2 = (mL * qL * vL + phL * pL * eny) * area;
+        r3 = (mL * qL * wL + phL * pL * enz) * area;
+        r4 = mL * qL * HL * area;
     } else if (SS > 0.0f) {
         const float idn = frcp((SS - SL) + 1.0e-6f);
         const float c1 = mL, c2 = mL * uL, c3 = mL * vL, c4 = mL * wL, c5 = mL * EL, d = qL - SL;
@@ -1624,11 +1626,11 @@ __device__ __forceinline__ void hllc_phase32(const Params &P, float phL, float r
         const float s3 = (d * c3 + phL * (pL - pst) * eny) * idn;
         const float s4 = (d * c4 + phL * (pL - pst) * enz) * idn;
         const float s5 = (d * c5 + phL * (pL * qL - pst * SS)) * idn;
-        F[0] = (mL * qL + SL * (s1 - c1)) * area;
-        F[1] = (mL * qL * uL + phL * pL * enx + SL * (s2 - c2)) * area;
-        F[2] = (mL * qL * vL + phL * pL * eny + SL * (s3 - c3)) * area;
-        F[3] = (mL * qL * wL + phL * pL * enz + SL * (s4 - c4)) * area;
-        F[4] = (mL * qL * HL + SL * (s5 - c5)) * area;
+        r0 = (mL * qL + SL * (s1 - c1)) * area;
+        r1 = (mL * qL * uL + phL * pL * enx + SL * (s2 - c2)) * area;
+        r2 = (mL * qL * vL + phL * pL * eny + SL * (s3 - c3)) * area;
+        r3 = (mL * qL * wL + phL * pL * enz + SL * (s4 - c4)) * area;
+        r4 = (mL * qL * HL + SL * (s5 - c5)) * area;
     } else if (SR >= 0.0f) {
         const float idn = frcp((SS - SR) + 1.0e-6f);
         const float c1 = mR, c2 = mR * uR, c3 = mR * vR, c4 = mR * wR, c5 = mR * ER, d = qR - SR;
@@ -1637,18 +1639,20 @@ __device__ __forceinline__ void hllc_phase32(const Params &P, float phL, float r
         const float s3 = (d * c3 + phR * (pR - pst) * eny) * idn;
         const float s4 = (d * c4 + phR * (pR - pst) * enz) * idn;
         const float s5 = (d * c5 + phR * (pR * qR - pst * SS)) * idn;
-        F[0] = (mR * qR + SR * (s1 - c1)) * area;
-        F[1] = (mR * qR * uR + phR * pR * enx + SR * (s2 - c2)) * area;
-        F[2] = (mR * qR * vR + phR * pR * eny + SR * (s3 - c3)) * area;
-        F[3] = (mR * qR * wR + phR * pR * enz + SR * (s4 - c4)) * area;
-        F[4] = (mR * qR * HR + SR * (s5 - c5)) * area;
+        r0 = (mR * qR + SR * (s1 - c1)) * area;
+        r1 = (mR * qR * uR + phR * pR * enx + SR * (s2 - c2)) * area;
+        r2 = (mR * qR * vR + phR * pR * eny + SR * (s3 - c3)) * area;
+        r3 = (mR * qR * wR + phR * pR * enz + SR * (s4 - c4)) * area;
+        r4 = (mR * qR * HR + SR * (s5 - c5)) * area;
     } else {
-        F[0] = mR * qR * area;
-        F[1] = (mR * qR * uR + phR * pR * enx) * area;
-        F[2] = (mR * qR * vR + phR * pR * eny) * area;
-        F[3] = (mR * qR * wR + phR * pR * enz) * area;
-        F[4] = mR * qR * HR * area;
+        r0 = mR * qR * area;
+        r1 = (mR * qR * uR + phR * pR * enx) * area;
+        r2 = (mR * qR * vR + phR * pR * eny) * area;
+        r3 = (mR * qR * wR + phR * pR * enz) * area;
+        r4 = mR * qR * HR * area;
     }
+    constexpr int o = GAS ? 0 : 5;
+    Fa[o] = r0; Fa[o + 1] = r1; Fa[o + 2] = r2; Fa[o + 3] = r3; Fa[o + 4] = r4;
 }
 
 // HLLC-Zien flux (sr_hllczien.F90:327-487) in FP32
@@ -1734,12 +1738,13 @@ __device__ __forceinline__ float quad_sum4f(float v, int qbase)
 }
 
 // interphase source (sr_source.F90:46-155) in FP32 for the FP32-bulk form:
-// differences of FP64 fields are formed in FP64; the source terms are
-// widened to FP64 where they join the FP64 residual, and the RK update is
-// the FP64 one
+// differences of FP64 fields are formed in FP64 (the face's centroid offset
+// dx, dy, dz and phig difference dph come in rounded from them); the source
+// terms are widened to FP64 where they join the FP64 residual, and the RK
+// update is the FP64 one
 __device__ __forceinline__ void source_and_update_f32(const Params &P, const State &s, const Mesh &m, int i,
-                                                      int f, int qbase, int neigh, const double *rhs, int stage,
-                                                      bool store, double (&cv)[3])
+                                                      int f, int qbase, float dx, float dy, float dz, float dph,
+                                                      const double *rhs, int stage, bool store, double (&cv)[3])
 {
 #pragma clang fp contract(on)
     const int ne = m.ne;
@@ -1769,9 +1774,6 @@ __device__ __forceinline__ void source_and_update_f32(const Params &P, const Sta
     const float wint = (phig * rog * wg + (1.0f - phig) * rop * wp) * iden;
     float a1, a2, a3, b1, b2, b3, c1, c2, c3;
     {
-        const float dx = (float)(m.xc[neigh] - m.xc[i]), dy = (float)(m.yc[neigh] - m.yc[i]),
-                    dz = (float)(m.zc[neigh] - m.zc[i]);
-        const float dph = (float)(QV(Q_PHIG, neigh) - phig_d);
         const float iw = frcp(fsqr(dx * dx + dy * dy + dz * dz));
         a1 = quad_sum4f(dx * dx * iw, qbase);
         a2 = quad_sum4f(dy * dy * iw, qbase);
@@ -1818,80 +1820,128 @@ __device__ __forceinline__ void rec_state32(const float (&r)[R32_N], FState32 &s
     st.up = r[7]; st.vp = r[8]; st.wp = r[9]; st.tp = r[10]; st.pg = r[11]; st.pp = r[12];
 }
 
-// face loop in FP32, one cell per quad of lanes as in k_flux; the quad's
-// face fluxes are widened to FP64 and summed in face order, then the FP64
-// source and stage update run unchanged.
+// The FP32 flux of face f of cell i (the face loop of hllcbatten /
+// hllczien): both sides' face states (the records' primitives, reconstructed
+// with the smaller limiter on interior faces), clipping and blending, then
+// the wall flux or the HLLC flux, into F.  Returns the neighbour.
 template <int FLUX>
-__device__ __forceinline__ void flux_update_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
-                                                int qbase, int stage, bool store, double (&cv)[3])
+__device__ __forceinline__ int face_flux_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
+                                             float (&F)[10])
 {
 #pragma clang fp contract(on)
     const int ne = m.ne;
     const double *q = s.q;
     const size_t nt = s.nt;
     const int neigh = m.nc[(size_t)f * ne + i];
-    float F[10];
-    {
-        const float scx = (float)m.sc[(size_t)(3 * f) * ne + i], scy = (float)m.sc[(size_t)(3 * f + 1) * ne + i],
-                    scz = (float)m.sc[(size_t)(3 * f + 2) * ne + i];
-        const float area = fsqr(scx * scx + scy * scy + scz * scz), ia = frcp(area);
-        const float enx = scx * ia, eny = scy * ia, enz = scz * ia;
-        const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
-        FState32 L, R;
-        if (P.myorder == 2) {
-            // both sides' records (cell i's and its neighbour's, a ghost's
-            // holding its primitives), twelve 16-byte loads each
-            float a[R32_N], b[R32_N];
-            const float4 *ra = reinterpret_cast<const float4 *>(s.rec32) + (size_t)(R32_N / 4) * i;
-            const float4 *rb = reinterpret_cast<const float4 *>(s.rec32) + (size_t)(R32_N / 4) * neigh;
+    const float scx = (float)m.sc[(size_t)(3 * f) * ne + i], scy = (float)m.sc[(size_t)(3 * f + 1) * ne + i],
+                scz = (float)m.sc[(size_t)(3 * f + 2) * ne + i];
+    const float area = fsqr(scx * scx + scy * scy + scz * scz), ia = frcp(area);
+    const float enx = scx * ia, eny = scy * ia, enz = scz * ia;
+    const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
+    FState32 L, R;
+    if (P.myorder == 2) {
+        // both sides' records (cell i's and its neighbour's, a ghost's
+        // holding its primitives), twelve 16-byte loads each
+        float a[R32_N], b[R32_N];
+        const float4 *ra = reinterpret_cast<const float4 *>(s.rec32) + (size_t)(R32_N / 4) * i;
+        const float4 *rb = reinterpret_cast<const float4 *>(s.rec32) + (size_t)(R32_N / 4) * neigh;
 #pragma unroll
-            for (int j = 0; j < R32_N / 4; j++) {
-                const float4 u = ra[j], v = rb[j];
-                a[4 * j] = u.x; a[4 * j + 1] = u.y; a[4 * j + 2] = u.z; a[4 * j + 3] = u.w;
-                b[4 * j] = v.x; b[4 * j + 1] = v.y; b[4 * j + 2] = v.z; b[4 * j + 3] = v.w;
-            }
-            rec_state32(a, L);
-            rec_state32(b, R);
-            if (neigh < ne) {
-                const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
-                             zf = m.fc0[(size_t)(3 * f + 2) * ne + i];
-                const float phimin = fmn(a[R32_PHI], b[R32_PHI]);
-                const float xf0 = (float)(xf - m.xc[i]), yf0 = (float)(yf - m.yc[i]), zf0 = (float)(zf - m.zc[i]);
-                const float xfn = (float)(xf - m.xc[neigh]), yfn = (float)(yf - m.yc[neigh]),
-                            zfn = (float)(zf - m.zc[neigh]);
-#define REC32(fld, sl)                                                                                   \
-    L.fld += phimin * (xf0 * a[R32_GRAD + 3 * (sl)] + yf0 * a[R32_GRAD + 3 * (sl) + 1] +                 \
-                       zf0 * a[R32_GRAD + 3 * (sl) + 2]);                                                \
-    R.fld += phimin * (xfn * b[R32_GRAD + 3 * (sl)] + yfn * b[R32_GRAD + 3 * (sl) + 1] +                 \
+        for (int j = 0; j < R32_N / 4; j++) {
+            const float4 u = ra[j], v = rb[j];
+            a[4 * j] = u.x; a[4 * j + 1] = u.y; a[4 * j + 2] = u.z; a[4 * j + 3] = u.w;
+            b[4 * j] = v.x; b[4 * j + 1] = v.y; b[4 * j + 2] = v.z; b[4 * j + 3] = v.w;
+        }
+        rec_state32(a, L);
+        rec_state32(b, R);
+        if (neigh < ne) {
+            const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
+                         zf = m.fc0[(size_t)(3 * f + 2) * ne + i];
+            const float phimin = fmn(a[R32_PHI], b[R32_PHI]);
+            const float xf0 = (float)(xf - m.xc[i]), yf0 = (float)(yf - m.yc[i]), zf0 = (float)(zf - m.zc[i]);
+            const float xfn = (float)(xf - m.xc[neigh]), yfn = (float)(yf - m.yc[neigh]),
+                        zfn = (float)(zf - m.zc[neigh]);
+#define REC32(fld, sl)                                                                               \
+    L.fld += phimin * (xf0 * a[R32_GRAD + 3 * (sl)] + yf0 * a[R32_GRAD + 3 * (sl) + 1] +             \
+                       zf0 * a[R32_GRAD + 3 * (sl) + 2]);                                            \
+    R.fld += phimin * (xfn * b[R32_GRAD + 3 * (sl)] + yfn * b[R32_GRAD + 3 * (sl) + 1] +             \
                        zfn * b[R32_GRAD + 3 * (sl) + 2]);
-                REC32(phig, G_PHIG) REC32(ug, G_UG) REC32(vg, G_VG) REC32(wg, G_WG) REC32(tg, G_TG)
-                REC32(pg, G_PG) REC32(up, G_UP) REC32(vp, G_VP) REC32(wp, G_WP) REC32(tp, G_TP) REC32(pp, G_PP)
+            REC32(phig, G_PHIG) REC32(ug, G_UG) REC32(vg, G_VG) REC32(wg, G_WG) REC32(tg, G_TG)
+            REC32(pg, G_PG) REC32(up, G_UP) REC32(vp, G_VP) REC32(wp, G_WP) REC32(tp, G_TP) REC32(pp, G_PP)
 #undef REC32
-            }
-        } else {
-            load_fstate32(q, nt, i, L);
-            load_fstate32(q, nt, neigh, R);
         }
-        L.phip = 1.0f - L.phig;
-        R.phip = 1.0f - R.phig;
-        clip_blend32(P, L);
-        clip_blend32(P, R);
-        if (ntype == 5) {
-            F[0] = 0.0f; F[4] = 0.0f; F[5] = 0.0f; F[9] = 0.0f;
-            F[1] = L.phig * L.pg * enx * area;
-            F[2] = L.phig * L.pg * eny * area;
-            F[3] = L.phig * L.pg * enz * area;
-            F[6] = L.phip * L.pp * enx * area;
-            F[7] = L.phip * L.pp * eny * area;
-            F[8] = L.phip * L.pp * enz * area;
-        } else if (FLUX == 1) {
-            hllc_zien32(P, L, R, enx, eny, enz, area, F);
-        } else {
-            hllc_phase32<true>(P, L.phig, L.rog, L.ug, L.vg, L.wg, L.tg, L.pg, R.phig, R.rog, R.ug, R.vg, R.wg,
-                               R.tg, R.pg, enx, eny, enz, area, F);
-            hllc_phase32<false>(P, L.phip, L.rop, L.up, L.vp, L.wp, L.tp, L.pp, R.phip, R.rop, R.up, R.vp,
-                                R.wp, R.tp, R.pp, enx, eny, enz, area, F + 5);
-        }
+    } else {
+        load_fstate32(q, nt, i, L);
+        load_fstate32(q, nt, neigh, R);
+    }
+    L.phip = 1.0f - L.phig;
+    R.phip = 1.0f - R.phig;
+    clip_blend32(P, L);
+    clip_blend32(P, R);
+    // the HLLC flux everywhere, the wall flux (pressure terms only) selected
+    // on wall faces: every component is written once, so F stays in registers
+    float G[10];
+    if (FLUX == 1) {
+        hllc_zien32(P, L, R, enx, eny, enz, area, G);
+    } else {
+        hllc_phase32<true>(P, L.phig, L.rog, L.ug, L.vg, L.wg, L.tg, L.pg, R.phig, R.rog, R.ug, R.vg, R.wg,
+                           R.tg, R.pg, enx, eny, enz, area, G);
+        hllc_phase32<false>(P, L.phip, L.rop, L.up, L.vp, L.wp, L.tp, L.pp, R.phip, R.rop, R.up, R.vp,
+                            R.wp, R.tp, R.pp, enx, eny, enz, area, G);
+    }
+    const bool wall = ntype == 5;
+    F[0] = wall ? 0.0f : G[0];
+    F[1] = wall ? L.phig * L.pg * enx * area : G[1];
+    F[2] = wall ? L.phig * L.pg * eny * area : G[2];
+    F[3] = wall ? L.phig * L.pg * enz * area : G[3];
+    F[4] = wall ? 0.0f : G[4];
+    F[5] = wall ? 0.0f : G[5];
+    F[6] = wall ? L.phip * L.pp * enx * area : G[6];
+    F[7] = wall ? L.phip * L.pp * eny * area : G[7];
+    F[8] = wall ? L.phip * L.pp * enz * area : G[8];
+    F[9] = wall ? 0.0f : G[9];
+    return neigh;
+}
+
+// the source's face terms: the centroid offset to the neighbour and the
+// phig difference, formed in FP64 and rounded (they change sign with the
+// face's orientation, exactly)
+__device__ __forceinline__ void face_offsets32(const State &s, const Mesh &m, int i, int neigh, float &dx,
+                                               float &dy, float &dz, float &dph)
+{
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    dx = (float)(m.xc[neigh] - m.xc[i]);
+    dy = (float)(m.yc[neigh] - m.yc[i]);
+    dz = (float)(m.zc[neigh] - m.zc[i]);
+    dph = (float)(QV(Q_PHIG, neigh) - QV(Q_PHIG, i));
+}
+
+// face loop in FP32, one cell per quad of lanes as in k_flux; the quad's
+// face fluxes are widened to FP64 and summed in face order, then the FP64
+// source and stage update run unchanged.  With HLLC-Batten a face's flux
+// (and the source's face terms) come from k_face32, which computed them once
+// for both cells: the higher-numbered cell takes it with the opposite sign
+// (the flux of the swapped face states across the reversed normal).
+template <int FLUX>
+__device__ __forceinline__ void flux_update_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
+                                                int qbase, int stage, bool store, double (&cv)[3])
+{
+#pragma clang fp contract(on)
+    float F[10];
+    float dx, dy, dz, dph;
+    if (FLUX == 2) {
+        // the face tables exist in every FP32-bulk HLLC-Batten context
+        const int code = s.fslot[(size_t)f * m.ne + i];
+        const float4 *src = reinterpret_cast<const float4 *>(s.fflux) + (size_t)FACE32_N / 4 * (code >> 1);
+        const float4 u0 = src[0], u1 = src[1], u2 = src[2], u3 = src[3];
+        const float sg = (code & 1) ? -1.0f : 1.0f;
+        F[0] = sg * u0.x; F[1] = sg * u0.y; F[2] = sg * u0.z; F[3] = sg * u0.w;
+        F[4] = sg * u1.x; F[5] = sg * u1.y; F[6] = sg * u1.z; F[7] = sg * u1.w;
+        F[8] = sg * u2.x; F[9] = sg * u2.y;
+        dph = sg * u2.z; dx = sg * u2.w; dy = sg * u3.x; dz = sg * u3.y;
+    } else {
+        const int neigh = face_flux_f32<FLUX>(P, s, m, i, f, F);
+        face_offsets32(s, m, i, neigh, dx, dy, dz, dph);
     }
     double rhs[3] = {0.0, 0.0, 0.0};
     for (int k = 0; k < 10; k++) {
@@ -1900,7 +1950,7 @@ __device__ __forceinline__ void flux_update_f32(const Params &P, const State &s,
         const double sum = 0.0 + t0 + t1 + t2 + t3;
         if ((k & 3) == f) rhs[k >> 2] = sum;
     }
-    source_and_update_f32(P, s, m, i, f, qbase, neigh, rhs, stage, store, cv);
+    source_and_update_f32(P, s, m, i, f, qbase, dx, dy, dz, dph, rhs, stage, store, cv);
 }
 
 // flux_face with the FP32-bulk face loop
@@ -2223,7 +2273,43 @@ __global__ void __launch_bounds__(4 * TILE) k_viscous(Batch B)
     }
 }
 
-// The FP32-bulk flux runs fastest held to three blocks per CU by a 48 KB
+// FP32-bulk HLLC-Batten: the flux of every face listed for the tile's
+// cells (its boundary faces and those towards higher-numbered cells; at
+// most 4 per cell), one face per lane, into the face's entry of fflux
+// (FACE32_N floats: the 10 components, the source's phig difference and
+// centroid offset), which k_flux then reads for both cells of the face
+// Held to three blocks per CU by a 48 KB static LDS reservation, as the
+// FP32 face loop of k_flux with HLLC-Zien: the compiler then keeps more
+// record loads in flight per wave (at 5.0M cells 0.82 ms against 1.18 ms
+// at 68 VGPRs, seven waves per SIMD).
+constexpr int FACE32_LDS_DOUBLES = 6 * 1024;
+template <int MODE>
+__global__ void __launch_bounds__(4 * TILE) k_face32(Batch B)
+{
+    __shared__ double resv[FACE32_LDS_DOUBLES];
+    if (B.n1 < 0) resv[threadIdx.x] = 0.0;  // never taken (n1 >= 0): keeps the reservation
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        const int base = C.S.fbase[C.first];
+        const int nf = C.S.fbase[C.first + C.count] - base;
+        const int l = (int)threadIdx.x;
+        if (l < nf) {
+            const int fc = C.S.flist[base + l], i = fc >> 2;
+            float F[10], dx, dy, dz, dph;
+            const int neigh = face_flux_f32<2>(C.P, C.S, C.M, i, fc & 3, F);
+            face_offsets32(C.S, C.M, i, neigh, dx, dy, dz, dph);
+            float4 *dst = reinterpret_cast<float4 *>(C.S.fflux) + (size_t)FACE32_N / 4 * (base + l);
+            dst[0] = make_float4(F[0], F[1], F[2], F[3]);
+            dst[1] = make_float4(F[4], F[5], F[6], F[7]);
+            dst[2] = make_float4(F[8], F[9], dph, dx);
+            dst[3] = make_float4(dy, dz, 0.0f, 0.0f);
+        }
+    }
+}
+
+// The FP32-bulk face loop (in k_flux with HLLC-Zien; k_face32 has its own)
+// runs fastest held to three blocks per CU by a 48 KB
 // static LDS reservation: the compiler then schedules with more registers
 // (more gathers in flight per wave).  At 4.8M cells: 2.22 ms with it (150
 // VGPRs), 3.01 ms without (109 VGPRs, four waves per SIMD); before the
@@ -2243,8 +2329,12 @@ static_assert(FLUX_STAGE_DOUBLES * 8 <= FLUX_F32_LDS_CAP, "staging fits the FP32
 template <int MODE, int FLUX, bool F32>
 __global__ void __launch_bounds__(4 * TILE) k_flux(Batch B, int stage, int feos)
 {
-    __shared__ double stage_lds[F32 ? FLUX_F32_LDS_CAP / 8 : FLUX_STAGE_DOUBLES];
-    if (F32 && stage < 0) stage_lds[threadIdx.x] = 0.0;  // never taken (stage >= 0): keeps the FP32 reservation
+    // the reservation serves the FP32 face loop run here (HLLC-Zien); with
+    // HLLC-Batten the faces run in k_face32 and this launch is the cells'
+    // sums, source and update, fastest unconstrained (1.47 against 1.76 ms)
+    constexpr bool RESV = F32 && FLUX != 2;
+    __shared__ double stage_lds[RESV ? FLUX_F32_LDS_CAP / 8 : FLUX_STAGE_DOUBLES];
+    if (RESV && stage < 0) stage_lds[threadIdx.x] = 0.0;  // never taken (stage >= 0): keeps the reservation
     TILE_LOOP(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
@@ -2395,7 +2485,11 @@ void launch_flux(const Batch &B, const Grid &G, int stage, int feos, hipStream_t
     const dim3 g(G.quad), b(4 * TILE);
     if (G.precision == 1) {
         if (G.myflux == 1) hipLaunchKernelGGL(PICK(2, k_flux, 1, true), g, b, 0, st, B, stage, feos);
-        else hipLaunchKernelGGL(PICK(2, k_flux, 2, true), g, b, 0, st, B, stage, feos);
+        else {
+            // each face's flux once, then the cells' sums, source and update
+            hipLaunchKernelGGL(PICK(2, k_face32), g, b, 0, st, B);
+            hipLaunchKernelGGL(PICK(2, k_flux, 2, true), g, b, 0, st, B, stage, feos);
+        }
     } else {
         if (G.myflux == 1) hipLaunchKernelGGL(PICK(2, k_flux, 1, false), g, b, 0, st, B, stage, feos);
         else hipLaunchKernelGGL(PICK(2, k_flux, 2, false), g, b, 0, st, B, stage, feos);
