@@ -1180,11 +1180,12 @@ __device__ __forceinline__ void source_and_update(const State &s, const Mesh &m,
 // cn, and in cu as well at the end of the step (`end`).  An overset donor
 // also keeps its stage update, unrewritten, in cnd: that is what the donor
 // gather after this launch interpolates.
+template <int STRIDE = TILE>
 __device__ __forceinline__ void eos_staged(const Params &P, const State &s, int i, const double *c_lds, bool end,
                                            double (&d)[4])
 {
     double c[10], v[Q_N], nv[10];
-    for (int k = 0; k < 10; k++) c[k] = c_lds[k * TILE];
+    for (int k = 0; k < 10; k++) c[k] = c_lds[k * STRIDE];
     eos_prims(P, c, v, nv);
     const int ne = s.ne;
     const size_t nt = s.nt;
@@ -1738,16 +1739,19 @@ __device__ __forceinline__ float quad_sum4f(float v, int qbase)
 }
 
 // interphase source (sr_source.F90:46-155) in FP32 for the FP32-bulk form:
-// differences of FP64 fields are formed in FP64 (the face's centroid offset
-// dx, dy, dz and phig difference dph come in rounded from them); the source
-// terms are widened to FP64 where they join the FP64 residual, and the RK
-// update is the FP64 one
-__device__ __forceinline__ void source_and_update_f32(const Params &P, const State &s, const Mesh &m, int i,
-                                                      int f, int qbase, float dx, float dy, float dz, float dph,
-                                                      const double *rhs, int stage, bool store, double (&cv)[3])
+// differences of FP64 fields are formed in FP64 (the faces' centroid offsets
+// dx, dy, dz and phig differences dph come in rounded from them); the
+// source terms are widened to FP64 where they join the FP64 residual, and
+// the RK update is the FP64 one.  Src32 holds cell i's terms from the sums
+// of its faces' phig LSQ (a1..c3).
+struct Src32 {
+    float fdgx, fdgy, fdgz, pint, dphx, dphy, dphz, work, phig, rog, phip, rop, grav;
+};
+__device__ __forceinline__ Src32 source_f32(const Params &P, const State &s, const Mesh &m, int i, float a1,
+                                            float a2, float a3, float b1, float b2, float b3, float c1, float c2,
+                                            float c3)
 {
 #pragma clang fp contract(on)
-    const int ne = m.ne;
     const double *q = s.q;
     const size_t nt = s.nt;
     const double phig_d = QV(Q_PHIG, i);
@@ -1757,21 +1761,69 @@ __device__ __forceinline__ void source_and_update_f32(const Params &P, const Sta
                  vp_d = QV(Q_VP, i), wp_d = QV(Q_WP, i), tg_d = QV(Q_TG, i), tp_d = QV(Q_TP, i);
     const float ug = (float)ug_d, vg = (float)vg_d, wg = (float)wg_d, up = (float)up_d, vp = (float)vp_d,
                 wp = (float)wp_d, tg = (float)tg_d, pg = (float)QV(Q_PG, i);
-    const double voli = m.vol[i];
     const float dl = (float)m.dl[i];
     const float amug = (float)P.as * fsqr(tg) * frcp(1.0f + (float)P.ts * frcp(tg));
     const float du = (float)(ug_d - up_d), dv = (float)(vg_d - vp_d), dw = (float)(wg_d - wp_d);
     const float q2 = du * du + dv * dv + dw * dw;
     const float drag = 0.44f * phig * (phig - 1.0f) * rog * fsqr(q2) * dl;
-    const float fdgx = drag * du, fdgy = drag * dv, fdgz = drag * dw;
+    Src32 r;
+    r.fdgx = drag * du;
+    r.fdgy = drag * dv;
+    r.fdgz = drag * dw;
     const float qsource = 6.0f * phip * amug * (float)P.cpg * 2.0f * (float)(tp_d - tg_d) *
                           frcp((float)P.prg * (dl * dl));
     const float delpstar = 2.0f * (phig * rog * phip * rop) * frcp(phig * rop + phip * rog) * q2;
-    const float pint = pg - fmn(delpstar, 0.01f * pg);
+    r.pint = pg - fmn(delpstar, 0.01f * pg);
     const float iden = frcp(phig * rog + (1.0f - phig) * rop);
     const float uint = (phig * rog * ug + (1.0f - phig) * rop * up) * iden;
     const float vint = (phig * rog * vg + (1.0f - phig) * rop * vp) * iden;
     const float wint = (phig * rog * wg + (1.0f - phig) * rop * wp) * iden;
+    const float idn = frcp(det3f(a1, b1, b2, b1, a2, b3, b2, b3, a3));
+    r.dphx = det3f(c1, c2, c3, b1, a2, b3, b2, b3, a3) * idn;
+    r.dphy = det3f(a1, b1, b2, c1, c2, c3, b2, b3, a3) * idn;
+    r.dphz = det3f(a1, b1, b2, b1, a2, b3, c1, c2, c3) * idn;
+    r.work = r.pint * (uint * r.dphx + vint * r.dphy + wint * r.dphz) + uint * r.fdgx + vint * r.fdgy +
+             wint * r.fdgz + qsource;
+    r.grav = (float)P.gravity;
+    r.phig = phig;
+    r.rog = rog;
+    r.phip = phip;
+    r.rop = rop;
+    return r;
+}
+
+// the stage update of conserved variable k of cell i from its face sum r:
+// the source term and viscous sum joined in FP64, then the RK stage
+__device__ __forceinline__ double update_f32(const State &s, int ne, int i, int k, double r, const Src32 &S,
+                                             double voli, int stage, double dtbyvol, bool store)
+{
+#pragma clang fp contract(on)
+    const double *vis = s.vis;
+    switch (k) {
+    case 1:
+        r = r - (double)(S.fdgx + S.pint * S.dphx + S.phig * S.rog * S.grav) * voli - vis[(size_t)0 * ne + i];
+        break;
+    case 2: r = r - (double)(S.fdgy + S.pint * S.dphy) * voli - vis[(size_t)1 * ne + i]; break;
+    case 3: r = r - (double)(S.fdgz + S.pint * S.dphz) * voli - vis[(size_t)2 * ne + i]; break;
+    case 4: r = r - (double)S.work * voli - vis[(size_t)3 * ne + i]; break;
+    case 6:
+        r = r + (double)(S.fdgx + S.pint * S.dphx - S.phip * S.rop * S.grav) * voli - vis[(size_t)4 * ne + i];
+        break;
+    case 7: r = r + (double)(S.fdgy + S.pint * S.dphy) * voli - vis[(size_t)5 * ne + i]; break;
+    case 8: r = r + (double)(S.fdgz + S.pint * S.dphz) * voli - vis[(size_t)6 * ne + i]; break;
+    case 9: r = r + (double)S.work * voli - vis[(size_t)7 * ne + i]; break;
+    default: break;
+    }
+    return rk_finish(s, (size_t)k * ne + i, r, stage, dtbyvol, store);
+}
+
+// the quad form: lane f holds face f's offsets and finishes the conserved
+// variables k = f, f+4, f+8 from their face sums rhs[0..2]
+__device__ __forceinline__ void source_and_update_f32(const Params &P, const State &s, const Mesh &m, int i,
+                                                      int f, int qbase, float dx, float dy, float dz, float dph,
+                                                      const double *rhs, int stage, bool store, double (&cv)[3])
+{
+#pragma clang fp contract(on)
     float a1, a2, a3, b1, b2, b3, c1, c2, c3;
     {
         const float iw = frcp(fsqr(dx * dx + dy * dy + dz * dz));
@@ -1785,31 +1837,63 @@ __device__ __forceinline__ void source_and_update_f32(const Params &P, const Sta
         c2 = quad_sum4f(dph * dy * iw, qbase);
         c3 = quad_sum4f(dph * dz * iw, qbase);
     }
-    const float idn = frcp(det3f(a1, b1, b2, b1, a2, b3, b2, b3, a3));
-    const float dphx = det3f(c1, c2, c3, b1, a2, b3, b2, b3, a3) * idn;
-    const float dphy = det3f(a1, b1, b2, c1, c2, c3, b2, b3, a3) * idn;
-    const float dphz = det3f(a1, b1, b2, b1, a2, b3, c1, c2, c3) * idn;
-    const float work = pint * (uint * dphx + vint * dphy + wint * dphz) + uint * fdgx + vint * fdgy +
-                       wint * fdgz + qsource;
-    const float grav = (float)P.gravity;
+    const Src32 S = source_f32(P, s, m, i, a1, a2, a3, b1, b2, b3, c1, c2, c3);
+    const double voli = m.vol[i];
     const double dtbyvol = (stage > 0) ? s.scal[SC_DT] / voli : 0.0;
     for (int j = 0; j < 3; j++) {
         const int k = f + 4 * j;
         if (k >= 10) break;
-        double r = rhs[j];
-        const double *vis = s.vis;
-        switch (k) {
-        case 1: r = r - (double)(fdgx + pint * dphx + phig * rog * grav) * voli - vis[(size_t)0 * ne + i]; break;
-        case 2: r = r - (double)(fdgy + pint * dphy) * voli - vis[(size_t)1 * ne + i]; break;
-        case 3: r = r - (double)(fdgz + pint * dphz) * voli - vis[(size_t)2 * ne + i]; break;
-        case 4: r = r - (double)work * voli - vis[(size_t)3 * ne + i]; break;
-        case 6: r = r + (double)(fdgx + pint * dphx - phip * rop * grav) * voli - vis[(size_t)4 * ne + i]; break;
-        case 7: r = r + (double)(fdgy + pint * dphy) * voli - vis[(size_t)5 * ne + i]; break;
-        case 8: r = r + (double)(fdgz + pint * dphz) * voli - vis[(size_t)6 * ne + i]; break;
-        case 9: r = r + (double)work * voli - vis[(size_t)7 * ne + i]; break;
-        default: break;
-        }
-        cv[j] = rk_finish(s, (size_t)k * ne + i, r, stage, dtbyvol, store);
+        cv[j] = update_f32(s, m.ne, i, k, rhs[j], S, voli, stage, dtbyvol, store);
+    }
+}
+
+// the cell form (HLLC-Batten, after k_face32): one thread per cell reads its
+// four face entries, sums the fluxes in face order (widened to FP64) and
+// the source's LSQ terms in face order, and finishes all ten conserved
+// variables
+__device__ __forceinline__ void cell_update_f32(const Params &P, const State &s, const Mesh &m, int i, int stage,
+                                                bool store, double (&cv)[10])
+{
+#pragma clang fp contract(on)
+    const int ne = m.ne;
+    float F[4][10], dx[4], dy[4], dz[4], dph[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const int code = s.fslot[(size_t)f * ne + i];
+        const float4 *src = reinterpret_cast<const float4 *>(s.fflux) + (size_t)FACE32_N / 4 * (code >> 1);
+        const float4 u0 = src[0], u1 = src[1], u2 = src[2], u3 = src[3];
+        const float sg = (code & 1) ? -1.0f : 1.0f;
+        F[f][0] = sg * u0.x; F[f][1] = sg * u0.y; F[f][2] = sg * u0.z; F[f][3] = sg * u0.w;
+        F[f][4] = sg * u1.x; F[f][5] = sg * u1.y; F[f][6] = sg * u1.z; F[f][7] = sg * u1.w;
+        F[f][8] = sg * u2.x; F[f][9] = sg * u2.y;
+        dph[f] = sg * u2.z; dx[f] = sg * u2.w; dy[f] = sg * u3.x; dz[f] = sg * u3.y;
+    }
+    // each face's LSQ terms as their own statements (no contraction across
+    // faces), then the sums in face order
+    float t[9][4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const float iw = frcp(fsqr(dx[f] * dx[f] + dy[f] * dy[f] + dz[f] * dz[f]));
+        t[0][f] = dx[f] * dx[f] * iw;
+        t[1][f] = dy[f] * dy[f] * iw;
+        t[2][f] = dz[f] * dz[f] * iw;
+        t[3][f] = dx[f] * dy[f] * iw;
+        t[4][f] = dx[f] * dz[f] * iw;
+        t[5][f] = dy[f] * dz[f] * iw;
+        t[6][f] = dph[f] * dx[f] * iw;
+        t[7][f] = dph[f] * dy[f] * iw;
+        t[8][f] = dph[f] * dz[f] * iw;
+    }
+    float sum[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) sum[j] = t[j][0] + t[j][1] + t[j][2] + t[j][3];
+    const Src32 S = source_f32(P, s, m, i, sum[0], sum[1], sum[2], sum[3], sum[4], sum[5], sum[6], sum[7], sum[8]);
+    const double voli = m.vol[i];
+    const double dtbyvol = (stage > 0) ? s.scal[SC_DT] / voli : 0.0;
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        const double r = 0.0 + (double)F[0][k] + (double)F[1][k] + (double)F[2][k] + (double)F[3][k];
+        cv[k] = update_f32(s, ne, i, k, r, S, voli, stage, dtbyvol, store);
     }
 }
 
@@ -1916,12 +2000,10 @@ __device__ __forceinline__ void face_offsets32(const State &s, const Mesh &m, in
     dph = (float)(QV(Q_PHIG, neigh) - QV(Q_PHIG, i));
 }
 
-// face loop in FP32, one cell per quad of lanes as in k_flux; the quad's
-// face fluxes are widened to FP64 and summed in face order, then the FP64
-// source and stage update run unchanged.  With HLLC-Batten a face's flux
-// (and the source's face terms) come from k_face32, which computed them once
-// for both cells: the higher-numbered cell takes it with the opposite sign
-// (the flux of the swapped face states across the reversed normal).
+// face loop in FP32, one cell per quad of lanes as in k_flux, each face
+// evaluated from the cell's side (HLLC-Zien; HLLC-Batten runs k_face32 and
+// k_cells32 instead); the quad's face fluxes are widened to FP64 and summed
+// in face order, then the FP64 source and stage update run unchanged.
 template <int FLUX>
 __device__ __forceinline__ void flux_update_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
                                                 int qbase, int stage, bool store, double (&cv)[3])
@@ -1929,20 +2011,8 @@ __device__ __forceinline__ void flux_update_f32(const Params &P, const State &s,
 #pragma clang fp contract(on)
     float F[10];
     float dx, dy, dz, dph;
-    if (FLUX == 2) {
-        // the face tables exist in every FP32-bulk HLLC-Batten context
-        const int code = s.fslot[(size_t)f * m.ne + i];
-        const float4 *src = reinterpret_cast<const float4 *>(s.fflux) + (size_t)FACE32_N / 4 * (code >> 1);
-        const float4 u0 = src[0], u1 = src[1], u2 = src[2], u3 = src[3];
-        const float sg = (code & 1) ? -1.0f : 1.0f;
-        F[0] = sg * u0.x; F[1] = sg * u0.y; F[2] = sg * u0.z; F[3] = sg * u0.w;
-        F[4] = sg * u1.x; F[5] = sg * u1.y; F[6] = sg * u1.z; F[7] = sg * u1.w;
-        F[8] = sg * u2.x; F[9] = sg * u2.y;
-        dph = sg * u2.z; dx = sg * u2.w; dy = sg * u3.x; dz = sg * u3.y;
-    } else {
-        const int neigh = face_flux_f32<FLUX>(P, s, m, i, f, F);
-        face_offsets32(s, m, i, neigh, dx, dy, dz, dph);
-    }
+    const int neigh = face_flux_f32<FLUX>(P, s, m, i, f, F);
+    face_offsets32(s, m, i, neigh, dx, dy, dz, dph);
     double rhs[3] = {0.0, 0.0, 0.0};
     for (int k = 0; k < 10; k++) {
         const double t0 = (double)__shfl(F[k], qbase, 64), t1 = (double)__shfl(F[k], qbase + 1, 64),
@@ -2308,6 +2378,38 @@ __global__ void __launch_bounds__(4 * TILE) k_face32(Batch B)
     }
 }
 
+// FP32-bulk HLLC-Batten after k_face32: one thread per cell sums its face
+// entries, adds the source and runs the stage update (cell_update_f32) and,
+// in the fused step (feos), the cell's primitive recovery (eos_staged), as
+// k_flux does for the four-lanes-per-cell forms
+template <int MODE>
+__global__ void __launch_bounds__(TILE) k_cells32(Batch B, int stage, int feos)
+{
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        const int l = (int)threadIdx.x;
+        double d[4] = {0.0, 0.0, 0.0, 0.0};
+        if (l < C.count) {
+            const int i = C.first + l, ne = C.M.ne;
+            double cv[10];
+            if (C.M.iblank[i] == 1) {
+                cell_update_f32(C.P, C.S, C.M, i, stage, feos == 0, cv);
+            } else if (feos) {
+                for (int k = 0; k < 10; k++) cv[k] = C.S.cn[(size_t)k * ne + i];
+            }
+            if (feos) eos_staged<1>(C.P, C.S, i, cv, feos == 2, d);
+        }
+        if (feos == 2) {
+            // the tile's monitor maxima (a max does not depend on the order)
+            for (int k = 0; k < 4; k++) {
+                const double v = wave_max(d[k]);
+                if (threadIdx.x == 0) B.part[(size_t)4 * t + k] = v;
+            }
+        }
+    }
+}
+
 // The FP32-bulk face loop (in k_flux with HLLC-Zien; k_face32 has its own)
 // runs fastest held to three blocks per CU by a 48 KB
 // static LDS reservation: the compiler then schedules with more registers
@@ -2329,12 +2431,8 @@ static_assert(FLUX_STAGE_DOUBLES * 8 <= FLUX_F32_LDS_CAP, "staging fits the FP32
 template <int MODE, int FLUX, bool F32>
 __global__ void __launch_bounds__(4 * TILE) k_flux(Batch B, int stage, int feos)
 {
-    // the reservation serves the FP32 face loop run here (HLLC-Zien); with
-    // HLLC-Batten the faces run in k_face32 and this launch is the cells'
-    // sums, source and update, fastest unconstrained (1.47 against 1.76 ms)
-    constexpr bool RESV = F32 && FLUX != 2;
-    __shared__ double stage_lds[RESV ? FLUX_F32_LDS_CAP / 8 : FLUX_STAGE_DOUBLES];
-    if (RESV && stage < 0) stage_lds[threadIdx.x] = 0.0;  // never taken (stage >= 0): keeps the reservation
+    __shared__ double stage_lds[F32 ? FLUX_F32_LDS_CAP / 8 : FLUX_STAGE_DOUBLES];
+    if (F32 && stage < 0) stage_lds[threadIdx.x] = 0.0;  // never taken (stage >= 0): keeps the FP32 reservation
     TILE_LOOP(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
@@ -2486,9 +2584,10 @@ void launch_flux(const Batch &B, const Grid &G, int stage, int feos, hipStream_t
     if (G.precision == 1) {
         if (G.myflux == 1) hipLaunchKernelGGL(PICK(2, k_flux, 1, true), g, b, 0, st, B, stage, feos);
         else {
-            // each face's flux once, then the cells' sums, source and update
+            // each face's flux once, then one thread per cell: its sums,
+            // source, update and primitive recovery
             hipLaunchKernelGGL(PICK(2, k_face32), g, b, 0, st, B);
-            hipLaunchKernelGGL(PICK(2, k_flux, 2, true), g, b, 0, st, B, stage, feos);
+            hipLaunchKernelGGL(PICK(1, k_cells32), dim3(G.cell), dim3(TILE), 0, st, B, stage, feos);
         }
     } else {
         if (G.myflux == 1) hipLaunchKernelGGL(PICK(2, k_flux, 1, false), g, b, 0, st, B, stage, feos);
