@@ -416,6 +416,7 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
         if (!hflist.empty()) {
             ALLOC(dfslot, hfslot.size()) ALLOC(dfbase, hfbase.size()) ALLOC(dflist, hflist.size())
             ALLOC(s.fflux, (size_t)FACE32_N * hflist.size())
+            ALLOC(s.fgeom, (size_t)12 * hflist.size())
         }
     } else {
         ALLOC(s.src, (size_t)SRC_N * ne)
@@ -447,6 +448,7 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
         return rc;
     }
     launch_geometry(m, xmax, ymax, zmax, ymin, zmin, c->st);
+    launch_face32_geometry(m, s, (int)hflist.size(), c->st);
     e = hipStreamSynchronize(c->st);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {
@@ -1245,7 +1247,7 @@ size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision)
     b += (30 + 3 * G_N + 1 + 8 + 10 + 10 + 2 + 10) * ne * D + 2 * (size_t)Q_N * nt * D + (SC_N + 4) * D + I;
     if (precision == 1) {
         b += (size_t)R32_N * nt * sizeof(float);  // FP32-bulk records
-        b += (9 * ne + 1) * I + 4 * ne * FACE32_N * sizeof(float);  // face tables and entries (HLLC-Batten; at most)
+        b += (9 * ne + 1) * I + 4 * ne * (FACE32_N + 12) * sizeof(float);  // face tables (HLLC-Batten; at most)
     }
     else b += SRC_N * ne * D;                                     // interphase source terms
     const size_t tc = (ne + TILE - 1) / TILE, tg = (ng + TILE - 1) / TILE + 1;

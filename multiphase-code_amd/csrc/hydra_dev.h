@@ -85,9 +85,11 @@ struct State {
     // components, the phig difference and centroid offset x, y, z towards
     // the other cell, 2 pad); fslot (4 x ne, face-major) gives each cell
     // face its entry as (k << 1) | 1 when the cell takes the listed values
-    // with the opposite sign
+    // with the opposite sign; fgeom holds each listed face's FP32 geometry,
+    // 12 floats (FaceGeo32 order: area vector, face-centroid offsets from
+    // both cells, the centroid offset between them), computed at set-up
     const int *fslot, *fbase, *flist;
-    float *fflux;
+    float *fflux, *fgeom;
     double *vis;            // 8 x ne
     double *d24, *d2q;      // 10 x ne
     double *anu;            // 2 x ne
@@ -146,6 +148,7 @@ struct Grid {
 
 void launch_geometry(const Mesh &m, double xmax, double ymax, double zmax, double ymin, double zmin,
                      hipStream_t st);
+void launch_face32_geometry(const Mesh &m, const State &s, int nf, hipStream_t st);
 void launch_eos(const Batch &B, const Grid &G, int want_dt, hipStream_t st);
 void launch_dt_final(const Batch &B, const Grid &G, hipStream_t st);
 void launch_bc(const Batch &B, const Grid &G, int viscous, hipStream_t st);

@@ -1904,21 +1904,52 @@ __device__ __forceinline__ void rec_state32(const float (&r)[R32_N], FState32 &s
     st.up = r[7]; st.vp = r[8]; st.wp = r[9]; st.tp = r[10]; st.pg = r[11]; st.pp = r[12];
 }
 
-// The FP32 flux of face f of cell i (the face loop of hllcbatten /
-// hllczien): both sides' face states (the records' primitives, reconstructed
-// with the smaller limiter on interior faces), clipping and blending, then
-// the wall flux or the HLLC flux, into F.  Returns the neighbour.
+// A face's geometry as the FP32 face loop uses it: the area vector rounded
+// from FP64, the face centroid's offsets from cell i's and the neighbour's
+// centroids and the neighbour's offset from cell i, each difference formed
+// in FP64 and rounded (the offsets to the face centroid only where a cell
+// lies on the far side)
+struct FaceGeo32 {
+    float scx, scy, scz, xf0, yf0, zf0, xfn, yfn, zfn, dx, dy, dz;
+};
+__device__ __forceinline__ FaceGeo32 face_geo32(const Mesh &m, int i, int f, int neigh)
+{
+    const int ne = m.ne;
+    FaceGeo32 g;
+    g.scx = (float)m.sc[(size_t)(3 * f) * ne + i];
+    g.scy = (float)m.sc[(size_t)(3 * f + 1) * ne + i];
+    g.scz = (float)m.sc[(size_t)(3 * f + 2) * ne + i];
+    g.xf0 = g.yf0 = g.zf0 = g.xfn = g.yfn = g.zfn = 0.0f;
+    if (neigh < ne) {
+        const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
+                     zf = m.fc0[(size_t)(3 * f + 2) * ne + i];
+        g.xf0 = (float)(xf - m.xc[i]);
+        g.yf0 = (float)(yf - m.yc[i]);
+        g.zf0 = (float)(zf - m.zc[i]);
+        g.xfn = (float)(xf - m.xc[neigh]);
+        g.yfn = (float)(yf - m.yc[neigh]);
+        g.zfn = (float)(zf - m.zc[neigh]);
+    }
+    g.dx = (float)(m.xc[neigh] - m.xc[i]);
+    g.dy = (float)(m.yc[neigh] - m.yc[i]);
+    g.dz = (float)(m.zc[neigh] - m.zc[i]);
+    return g;
+}
+
+// The FP32 flux of cell i's face towards neigh (the face loop of
+// hllcbatten / hllczien) with the face's geometry g: both sides' face states
+// (the records' primitives, reconstructed with the smaller limiter on
+// interior faces), clipping and blending, then the wall flux or the HLLC
+// flux, into F.
 template <int FLUX>
-__device__ __forceinline__ int face_flux_f32(const Params &P, const State &s, const Mesh &m, int i, int f,
-                                             float (&F)[10])
+__device__ __forceinline__ void face_flux_f32(const Params &P, const State &s, const Mesh &m, int i, int neigh,
+                                              const FaceGeo32 &g, float (&F)[10])
 {
 #pragma clang fp contract(on)
     const int ne = m.ne;
     const double *q = s.q;
     const size_t nt = s.nt;
-    const int neigh = m.nc[(size_t)f * ne + i];
-    const float scx = (float)m.sc[(size_t)(3 * f) * ne + i], scy = (float)m.sc[(size_t)(3 * f + 1) * ne + i],
-                scz = (float)m.sc[(size_t)(3 * f + 2) * ne + i];
+    const float scx = g.scx, scy = g.scy, scz = g.scz;
     const float area = fsqr(scx * scx + scy * scy + scz * scz), ia = frcp(area);
     const float enx = scx * ia, eny = scy * ia, enz = scz * ia;
     const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
@@ -1938,12 +1969,8 @@ __device__ __forceinline__ int face_flux_f32(const Params &P, const State &s, co
         rec_state32(a, L);
         rec_state32(b, R);
         if (neigh < ne) {
-            const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
-                         zf = m.fc0[(size_t)(3 * f + 2) * ne + i];
             const float phimin = fmn(a[R32_PHI], b[R32_PHI]);
-            const float xf0 = (float)(xf - m.xc[i]), yf0 = (float)(yf - m.yc[i]), zf0 = (float)(zf - m.zc[i]);
-            const float xfn = (float)(xf - m.xc[neigh]), yfn = (float)(yf - m.yc[neigh]),
-                        zfn = (float)(zf - m.zc[neigh]);
+            const float xf0 = g.xf0, yf0 = g.yf0, zf0 = g.zf0, xfn = g.xfn, yfn = g.yfn, zfn = g.zfn;
 #define REC32(fld, sl)                                                                               \
     L.fld += phimin * (xf0 * a[R32_GRAD + 3 * (sl)] + yf0 * a[R32_GRAD + 3 * (sl) + 1] +             \
                        zf0 * a[R32_GRAD + 3 * (sl) + 2]);                                            \
@@ -1983,21 +2010,16 @@ __device__ __forceinline__ int face_flux_f32(const Params &P, const State &s, co
     F[7] = wall ? L.phip * L.pp * eny * area : G[7];
     F[8] = wall ? L.phip * L.pp * enz * area : G[8];
     F[9] = wall ? 0.0f : G[9];
-    return neigh;
 }
 
-// the source's face terms: the centroid offset to the neighbour and the
-// phig difference, formed in FP64 and rounded (they change sign with the
-// face's orientation, exactly)
-__device__ __forceinline__ void face_offsets32(const State &s, const Mesh &m, int i, int neigh, float &dx,
-                                               float &dy, float &dz, float &dph)
+// the source's phig difference across the face, formed in FP64 and rounded
+// (it changes sign with the face's orientation, exactly, as the centroid
+// offset does)
+__device__ __forceinline__ float face_dphig32(const State &s, int i, int neigh)
 {
     const double *q = s.q;
     const size_t nt = s.nt;
-    dx = (float)(m.xc[neigh] - m.xc[i]);
-    dy = (float)(m.yc[neigh] - m.yc[i]);
-    dz = (float)(m.zc[neigh] - m.zc[i]);
-    dph = (float)(QV(Q_PHIG, neigh) - QV(Q_PHIG, i));
+    return (float)(QV(Q_PHIG, neigh) - QV(Q_PHIG, i));
 }
 
 // face loop in FP32, one cell per quad of lanes as in k_flux, each face
@@ -2010,9 +2032,10 @@ __device__ __forceinline__ void flux_update_f32(const Params &P, const State &s,
 {
 #pragma clang fp contract(on)
     float F[10];
-    float dx, dy, dz, dph;
-    const int neigh = face_flux_f32<FLUX>(P, s, m, i, f, F);
-    face_offsets32(s, m, i, neigh, dx, dy, dz, dph);
+    const int neigh = m.nc[(size_t)f * m.ne + i];
+    const FaceGeo32 g = face_geo32(m, i, f, neigh);
+    face_flux_f32<FLUX>(P, s, m, i, neigh, g, F);
+    const float dx = g.dx, dy = g.dy, dz = g.dz, dph = face_dphig32(s, i, neigh);
     double rhs[3] = {0.0, 0.0, 0.0};
     for (int k = 0; k < 10; k++) {
         const double t0 = (double)__shfl(F[k], qbase, 64), t1 = (double)__shfl(F[k], qbase + 1, 64),
@@ -2366,9 +2389,13 @@ __global__ void __launch_bounds__(4 * TILE) k_face32(Batch B)
         const int l = (int)threadIdx.x;
         if (l < nf) {
             const int fc = C.S.flist[base + l], i = fc >> 2;
-            float F[10], dx, dy, dz, dph;
-            const int neigh = face_flux_f32<2>(C.P, C.S, C.M, i, fc & 3, F);
-            face_offsets32(C.S, C.M, i, neigh, dx, dy, dz, dph);
+            const int neigh = C.M.nc[(size_t)(fc & 3) * C.M.ne + i];
+            const float4 *gs = reinterpret_cast<const float4 *>(C.S.fgeom) + (size_t)3 * (base + l);
+            const float4 g0 = gs[0], g1 = gs[1], g2 = gs[2];
+            const FaceGeo32 g = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w, g2.x, g2.y, g2.z, g2.w};
+            float F[10];
+            face_flux_f32<2>(C.P, C.S, C.M, i, neigh, g, F);
+            const float dph = face_dphig32(C.S, i, neigh), dx = g.dx, dy = g.dy, dz = g.dz;
             float4 *dst = reinterpret_cast<float4 *>(C.S.fflux) + (size_t)FACE32_N / 4 * (base + l);
             dst[0] = make_float4(F[0], F[1], F[2], F[3]);
             dst[1] = make_float4(F[4], F[5], F[6], F[7]);
@@ -2376,6 +2403,24 @@ __global__ void __launch_bounds__(4 * TILE) k_face32(Batch B)
             dst[3] = make_float4(dy, dz, 0.0f, 0.0f);
         }
     }
+}
+
+// the FP32-bulk HLLC-Batten face table's geometry (State::fgeom), once
+__global__ void k_face32_geom(Mesh m, State s, int nf)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nf) return;
+    const int fc = s.flist[k], i = fc >> 2, f = fc & 3;
+    const FaceGeo32 g = face_geo32(m, i, f, m.nc[(size_t)f * m.ne + i]);
+    float4 *d = reinterpret_cast<float4 *>(s.fgeom) + (size_t)3 * k;
+    d[0] = make_float4(g.scx, g.scy, g.scz, g.xf0);
+    d[1] = make_float4(g.yf0, g.zf0, g.xfn, g.yfn);
+    d[2] = make_float4(g.zfn, g.dx, g.dy, g.dz);
+}
+
+void launch_face32_geometry(const Mesh &m, const State &s, int nf, hipStream_t st)
+{
+    if (nf > 0) hipLaunchKernelGGL(k_face32_geom, dim3(nblk(nf, 256)), dim3(256), 0, st, m, s, nf);
 }
 
 // FP32-bulk HLLC-Batten after k_face32: one thread per cell sums its face
