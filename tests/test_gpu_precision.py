@@ -31,6 +31,7 @@ quantity's scale.
 """
 import hashlib
 import json
+import math
 import os
 
 import numpy as np
@@ -199,3 +200,28 @@ def test_fp32_bulk_one_stage_against_fp64():
 
 
 STAGE_RHS_LIQUID = 1e-4
+
+
+def test_fp32_bulk_faces_conserve_mass():
+    """With HLLC-Batten the FP32-bulk form computes each face's flux once and
+    gives it to both cells with opposite signs, so on a closed box (every
+    boundary face a slip wall, whose mass flux is zero) the phase-mass
+    residuals of one stage sum to zero up to the FP64 rounding of each
+    cell's four-term sum (math.fsum adds the cells exactly).  HLLC-Zien,
+    which evaluates every face from both of its cells as the reference
+    does, leaves an imbalance of FP32 roundoff size: the same check fails
+    for it, so it can tell the two apart."""
+    m = M.box_mesh(12, 12, 12, jitter=0.1, seed=5)
+    m.gtype[:] = M.BC_WALL
+    p = M.water_air_params(cfl=0.5, viscous_gas=True)
+    cu, gp = M.sphere_state(m, p, [(0.1, 0.0, 0.0, 0.2)])
+    for myflux in (2, 1):
+        r = stage_outputs(m, dict(p, myflux=myflux), cu, gp, 1)["rhs"]
+        for k in (0, 5):  # gas and liquid mass: face fluxes only
+            imbalance = abs(math.fsum(r[k])) / math.fsum(np.abs(r[k]))
+            print(f"myflux {myflux} variable {k}: mass imbalance {imbalance:.2e}")
+            if myflux == 2:
+                assert imbalance <= 1e-13, (k, imbalance)
+            else:
+                assert imbalance > 1e-11, (k, imbalance)
+
