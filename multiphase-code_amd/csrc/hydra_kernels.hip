@@ -158,6 +158,23 @@ __device__ __forceinline__ T quad_plane(const T (&v)[N])
     return quad_lane<P & 3>(v[P >> 2]);
 }
 
+// The row layout of the viscous kernels: the wave's 16 cells c hold their
+// four faces in lanes c, c+16, c+32, c+48, so a load instruction's lanes
+// take one face of 16 consecutive cells (measured faster there; the flux
+// keeps quads, whose many shared planes move by DPP).  Lane J of this
+// lane's cell by a lane shuffle.
+template <int J>
+__device__ __forceinline__ double row_lane(double v)
+{
+    return __shfl(v, (int)(threadIdx.x & 15) + 16 * J, 64);
+}
+template <int P, int N>
+__device__ __forceinline__ double row_plane(const double (&v)[N])
+{
+    static_assert(P >= 0 && (P >> 2) < N, "plane");
+    return row_lane<P & 3>(v[P >> 2]);
+}
+
 // ---------------------------------------------------------------------------
 // geometry (sr_geometry.F90:147-289), once per mesh
 // ---------------------------------------------------------------------------
@@ -559,15 +576,15 @@ __device__ __forceinline__ void smooth2_cell(const Params &P, const State &s, co
 // plus the cell (boundary face) or plus the neighbour's 4 neighbours
 // (interior face); Newtonian stress and Fourier heat flux of both phases.
 // ---------------------------------------------------------------------------
-// quad_lane with a lane index that is a constant once the caller's loop is
+// row_lane with a lane index that is a constant once the caller's loop is
 // unrolled
-__device__ __forceinline__ double quad_from(double v, int k)
+__device__ __forceinline__ double row_from(double v, int k)
 {
     switch (k) {
-    case 0: return quad_lane<0>(v);
-    case 1: return quad_lane<1>(v);
-    case 2: return quad_lane<2>(v);
-    default: return quad_lane<3>(v);
+    case 0: return row_lane<0>(v);
+    case 1: return row_lane<1>(v);
+    case 2: return row_lane<2>(v);
+    default: return row_lane<3>(v);
     }
 }
 
@@ -576,10 +593,11 @@ __device__ __forceinline__ double quad_from(double v, int k)
 // 450-804).  The stencil of an interior face is the four face neighbours of
 // cell i, then the four of the cell across face f (m2); that of a boundary
 // face is cell i's neighbours and cell i itself (m2[0] = i).  Points 0-3 are
-// the same for the four faces of the quad: lane k holds point k's centroid
-// (xn, yn, zn) and primitives (qn) and hands them over by DPP, so the four
-// lanes run the stencil together, and a boundary-face lane drops points 5-7
-// by selects, leaving every sum the five-point one.
+// the same for the cell's four faces: the lane of face k holds point k's
+// centroid (xn, yn, zn) and primitives (qn) and hands them over by lane
+// shuffles (row_from), so the four lanes run the stencil together, and a
+// boundary-face lane drops points 5-7 by selects, leaving every sum the
+// five-point one.
 __device__ __forceinline__ void viscous_stencil(const Params &P, const double *q, size_t nt, const Mesh &m,
                                                 const int (&m2)[4], bool ghost, double xn, double yn, double zn,
                                                 const double (&qn)[8], double xf, double yf, double zf,
@@ -595,9 +613,9 @@ __device__ __forceinline__ void viscous_stencil(const Params &P, const double *q
     double dx[8], dy[8], dz[8];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        dx[k] = quad_from(xn, k) - xf;
-        dy[k] = quad_from(yn, k) - yf;
-        dz[k] = quad_from(zn, k) - zf;
+        dx[k] = row_from(xn, k) - xf;
+        dy[k] = row_from(yn, k) - yf;
+        dz[k] = row_from(zn, k) - zf;
     }
 #pragma unroll
     for (int k = 4; k < 8; k++) {
@@ -622,12 +640,12 @@ __device__ __forceinline__ void viscous_stencil(const Params &P, const double *q
         double gr[4][3];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            double dv = quad_from(qn[o + v], 0) - fv[o + v];
+            double dv = row_from(qn[o + v], 0) - fv[o + v];
             double b1 = dv * dx[0], b2 = dv * dy[0], b3 = dv * dz[0];
 #pragma unroll
             for (int k = 1; k < 8; k++) {
                 const bool use = !(ghost && k >= 5);
-                const double qk = k < 4 ? quad_from(qn[o + v], k) : QV(slot[o + v], m2[k - 4]);
+                const double qk = k < 4 ? row_from(qn[o + v], k) : QV(slot[o + v], m2[k - 4]);
                 dv = qk - fv[o + v];
                 const double n1 = b1 + dv * dx[k], n2 = b2 + dv * dy[k], n3 = b3 + dv * dz[k];
                 b1 = use ? n1 : b1; b2 = use ? n2 : b2; b3 = use ? n3 : b3;
@@ -686,16 +704,16 @@ __device__ __forceinline__ void viscous_face(const Params &P, const State &s, co
         double vq[(Q_N + 3) / 4];
         quad_load<Q_N>(q, nt, i, f, vq);
         double fv[8];
-        fv[0] = (quad_plane<Q_UG>(vq) + qn[0]) * 0.5;
-        fv[1] = (quad_plane<Q_VG>(vq) + qn[1]) * 0.5;
-        fv[2] = (quad_plane<Q_WG>(vq) + qn[2]) * 0.5;
-        fv[3] = (quad_plane<Q_TG>(vq) + qn[3]) * 0.5;
-        fv[4] = (quad_plane<Q_UP>(vq) + qn[4]) * 0.5;
-        fv[5] = (quad_plane<Q_VP>(vq) + qn[5]) * 0.5;
-        fv[6] = (quad_plane<Q_WP>(vq) + qn[6]) * 0.5;
-        fv[7] = (quad_plane<Q_TP>(vq) + qn[7]) * 0.5;
-        const double phigf = (quad_plane<Q_PHIG>(vq) + QV(Q_PHIG, neigh)) * 0.5;
-        const double phipf = (quad_plane<Q_PHIP>(vq) + QV(Q_PHIP, neigh)) * 0.5;
+        fv[0] = (row_plane<Q_UG>(vq) + qn[0]) * 0.5;
+        fv[1] = (row_plane<Q_VG>(vq) + qn[1]) * 0.5;
+        fv[2] = (row_plane<Q_WG>(vq) + qn[2]) * 0.5;
+        fv[3] = (row_plane<Q_TG>(vq) + qn[3]) * 0.5;
+        fv[4] = (row_plane<Q_UP>(vq) + qn[4]) * 0.5;
+        fv[5] = (row_plane<Q_VP>(vq) + qn[5]) * 0.5;
+        fv[6] = (row_plane<Q_WP>(vq) + qn[6]) * 0.5;
+        fv[7] = (row_plane<Q_TP>(vq) + qn[7]) * 0.5;
+        const double phigf = (row_plane<Q_PHIG>(vq) + QV(Q_PHIG, neigh)) * 0.5;
+        const double phipf = (row_plane<Q_PHIP>(vq) + QV(Q_PHIP, neigh)) * 0.5;
         const bool ghost = neigh >= ne;
         const int far = ghost ? i : neigh;
         int m2[4];
@@ -703,8 +721,8 @@ __device__ __forceinline__ void viscous_face(const Params &P, const State &s, co
         viscous_stencil(P, q, nt, m, m2, ghost, xn, yn, zn, qn, xf, yf, zf, scx, scy, scz, fv, phigf, phipf, acc);
     }
     for (int v = 0; v < 8; v++) {
-        const double t0 = quad_get(acc[v], qbase, 0), t1 = quad_get(acc[v], qbase, 1),
-                     t2 = quad_get(acc[v], qbase, 2), t3 = quad_get(acc[v], qbase, 3);
+        const double t0 = __shfl(acc[v], qbase, 64), t1 = __shfl(acc[v], qbase + 16, 64),
+                     t2 = __shfl(acc[v], qbase + 32, 64), t3 = __shfl(acc[v], qbase + 48, 64);
         if ((v & 3) == f) s.vis[(size_t)v * ne + i] = 0.0 + t0 + t1 + t2 + t3;
     }
 }
@@ -2085,9 +2103,9 @@ __device__ __forceinline__ void viscous_stencil_f32(const Params &P, const doubl
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const bool use = !(ghost && k >= 5);
-        const double px = k < 4 ? quad_from(xn, k) : m.xc[m2[k - 4]];
-        const double py = k < 4 ? quad_from(yn, k) : m.yc[m2[k - 4]];
-        const double pz = k < 4 ? quad_from(zn, k) : m.zc[m2[k - 4]];
+        const double px = k < 4 ? row_from(xn, k) : m.xc[m2[k - 4]];
+        const double py = k < 4 ? row_from(yn, k) : m.yc[m2[k - 4]];
+        const double pz = k < 4 ? row_from(zn, k) : m.zc[m2[k - 4]];
         dx[k] = use ? (float)(px - xf) : 0.0f;
         dy[k] = use ? (float)(py - yf) : 0.0f;
         dz[k] = use ? (float)(pz - zf) : 0.0f;
@@ -2108,7 +2126,7 @@ __device__ __forceinline__ void viscous_stencil_f32(const Params &P, const doubl
             float b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const double qk = k < 4 ? quad_from(qn[o + v], k) : QV(slot[o + v], m2[k - 4]);
+                const double qk = k < 4 ? row_from(qn[o + v], k) : QV(slot[o + v], m2[k - 4]);
                 const float dv = (float)(qk - fv[o + v]);
                 b1 += dv * dx[k];
                 b2 += dv * dy[k];
@@ -2160,16 +2178,16 @@ __device__ __forceinline__ void viscous_face_f32(const Params &P, const State &s
         double vq[(Q_N + 3) / 4];
         quad_load<Q_N>(q, nt, i, f, vq);
         double fv[8];
-        fv[0] = (quad_plane<Q_UG>(vq) + qn[0]) * 0.5;
-        fv[1] = (quad_plane<Q_VG>(vq) + qn[1]) * 0.5;
-        fv[2] = (quad_plane<Q_WG>(vq) + qn[2]) * 0.5;
-        fv[3] = (quad_plane<Q_TG>(vq) + qn[3]) * 0.5;
-        fv[4] = (quad_plane<Q_UP>(vq) + qn[4]) * 0.5;
-        fv[5] = (quad_plane<Q_VP>(vq) + qn[5]) * 0.5;
-        fv[6] = (quad_plane<Q_WP>(vq) + qn[6]) * 0.5;
-        fv[7] = (quad_plane<Q_TP>(vq) + qn[7]) * 0.5;
-        const float phigf = (float)((quad_plane<Q_PHIG>(vq) + QV(Q_PHIG, neigh)) * 0.5);
-        const float phipf = (float)((quad_plane<Q_PHIP>(vq) + QV(Q_PHIP, neigh)) * 0.5);
+        fv[0] = (row_plane<Q_UG>(vq) + qn[0]) * 0.5;
+        fv[1] = (row_plane<Q_VG>(vq) + qn[1]) * 0.5;
+        fv[2] = (row_plane<Q_WG>(vq) + qn[2]) * 0.5;
+        fv[3] = (row_plane<Q_TG>(vq) + qn[3]) * 0.5;
+        fv[4] = (row_plane<Q_UP>(vq) + qn[4]) * 0.5;
+        fv[5] = (row_plane<Q_VP>(vq) + qn[5]) * 0.5;
+        fv[6] = (row_plane<Q_WP>(vq) + qn[6]) * 0.5;
+        fv[7] = (row_plane<Q_TP>(vq) + qn[7]) * 0.5;
+        const float phigf = (float)((row_plane<Q_PHIG>(vq) + QV(Q_PHIG, neigh)) * 0.5);
+        const float phipf = (float)((row_plane<Q_PHIP>(vq) + QV(Q_PHIP, neigh)) * 0.5);
         const bool ghost = neigh >= ne;
         const int far = ghost ? i : neigh;
         int m2[4];
@@ -2178,8 +2196,8 @@ __device__ __forceinline__ void viscous_face_f32(const Params &P, const State &s
                             phipf, acc);
     }
     for (int v = 0; v < 8; v++) {
-        const double t0 = (double)__shfl(acc[v], qbase, 64), t1 = (double)__shfl(acc[v], qbase + 1, 64),
-                     t2 = (double)__shfl(acc[v], qbase + 2, 64), t3 = (double)__shfl(acc[v], qbase + 3, 64);
+        const double t0 = (double)__shfl(acc[v], qbase, 64), t1 = (double)__shfl(acc[v], qbase + 16, 64),
+                     t2 = (double)__shfl(acc[v], qbase + 32, 64), t3 = (double)__shfl(acc[v], qbase + 48, 64);
         if ((v & 3) == f) s.vis[(size_t)v * ne + i] = 0.0 + t0 + t1 + t2 + t3;
     }
 }
@@ -2349,14 +2367,16 @@ __global__ void __launch_bounds__((!F32 && GRAD) ? 2 * TILE : TILE) k_grad_limit
     }
 }
 
-// lanes 4l..4l+3 of a block own cell C.first + l, one face each
+// four lanes of a block per cell C.first + l, one face each, in rows
 template <int MODE, bool F32>
 __global__ void __launch_bounds__(4 * TILE) k_viscous(Batch B)
 {
     TILE_LOOP(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
-        const int l = (int)(threadIdx.x >> 2), f = (int)(threadIdx.x & 3), qbase = (int)(threadIdx.x & 63) & ~3;
+        // row layout: lane c + 16 f of wave w holds face f of cell 16 w + c
+        const int l = (int)((threadIdx.x >> 6) * 16 + (threadIdx.x & 15)), f = (int)((threadIdx.x >> 4) & 3),
+                  qbase = (int)(threadIdx.x & 15);
         if (l < C.count) {
             if (F32)
                 viscous_face_f32(C.P, C.S, C.M, C.first + l, f, qbase);
