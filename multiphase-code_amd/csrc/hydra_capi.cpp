@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "hydra_dev.h"
+#include "hydra_faces.h"
 #include "hydra_hip.h"
 
 using namespace hydra;
@@ -322,47 +323,12 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
         const int b = hnc[k];
         hncp[k] = b < ne ? b : gpar[b - ne];
     }
-    // FP32-bulk HLLC-Batten: the face tables of State::fslot/fbase/flist.  A
-    // cell lists its boundary faces and the faces towards higher-numbered
-    // cells; the other cell of a listed face takes that entry, negated.  A
-    // pair of cells joined by more than one face (not a valid mesh) keeps a
-    // flux per side.
+    // FP32-bulk HLLC-Batten: the face tables of State::fslot/fbase/flist
+    // (each face computed once, hydra_faces.h)
     std::vector<int> hfslot, hfbase, hflist;
     if (params->precision == 1 && params->myflux == 2) {
         if (ne >= (1 << 28)) return fail(nullptr, HP_ERR_ARG, "hp_create: FP32-bulk face tables take < 2^28 cells");
-        hfslot.assign(4 * (size_t)ne, 0);
-        hfbase.assign((size_t)ne + 1, 0);
-        hflist.reserve(3 * (size_t)ne);
-        // the face of j towards i when i and j share exactly one face, else -1
-        auto back = [&](int i, int j) -> int {
-            int nij = 0, nji = 0, fb = -1;
-            for (int k = 0; k < 4; k++) {
-                if (hnc[(size_t)k * ne + i] == j) nij++;
-                if (hnc[(size_t)k * ne + j] == i) {
-                    nji++;
-                    fb = k;
-                }
-            }
-            return (nij == 1 && nji == 1) ? fb : -1;
-        };
-        for (int i = 0; i < ne; i++) {
-            hfbase[i] = (int)hflist.size();
-            for (int f = 0; f < 4; f++) {
-                const int j = hnc[(size_t)f * ne + i];
-                if (j >= ne || i < j || back(i, j) < 0) {
-                    hfslot[(size_t)f * ne + i] = (int)hflist.size() << 1;
-                    hflist.push_back(4 * i + f);
-                }
-            }
-        }
-        hfbase[ne] = (int)hflist.size();
-        for (int i = 0; i < ne; i++)
-            for (int f = 0; f < 4; f++) {
-                const int j = hnc[(size_t)f * ne + i];
-                if (j >= ne || i < j) continue;
-                const int fb = back(i, j);
-                if (fb >= 0) hfslot[(size_t)f * ne + i] = hfslot[(size_t)fb * ne + j] | 1;
-            }
+        build_face_tables(ne, hnc.data(), hfslot, hfbase, hflist);
     }
     double xmax = -1.0e30, ymax = -1.0e30, zmax = -1.0e30, ymin = 1.0e30, zmin = 1.0e30;
     for (int n = 0; n < nodes; n++) {
