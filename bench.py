@@ -205,7 +205,7 @@ def summarize_counters(read_csv, write_csv, calib_mb):
           f"{wfac:.4f} (write)\n")
     print("| kernel | dispatches | read req 128/64/32 B (share of requests) | HBM read MB | HBM write MB |")
     print("|---|---|---|---|---|")
-    flux = None
+    flux = face32 = None
     for k in rd:
         if "hydra" not in k or "k_stream_copy" in k:
             continue
@@ -218,11 +218,15 @@ def summarize_counters(read_csv, write_csv, calib_mb):
         print(f"| `{name}` | {nrd[k]} | {mix} | {r:.1f} | {w:.1f} |")
         if name.startswith("hydra::k_flux"):
             flux = int(round((r + w) * 1e6))
+        if name.startswith("hydra::k_face32") or name.startswith("hydra::k_cells32"):
+            face32 = (face32 or 0) + int(round((r + w) * 1e6))
     if flux is not None:
         print(f"\ntraffic_bytes_per_launch k_flux: {flux}")
+    if face32 is not None:
+        print(f"\ntraffic_bytes_per_stage k_face32 + k_cells32: {face32}")
 
 
-def summarize_pmc(path, kernels=("k_flux", "k_grad", "k_viscous", "k_stream_copy")):
+def summarize_pmc(path, kernels=("k_flux", "k_face32<", "k_cells32", "k_grad", "k_viscous", "k_stream_copy")):
     """Mean per dispatch of every counter of one rocprofv3 --pmc run, for the
     kernels named (substrings).  Prints markdown."""
     means, nd = _pmc_means(path)
