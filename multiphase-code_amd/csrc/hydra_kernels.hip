@@ -175,6 +175,56 @@ __device__ __forceinline__ double row_plane(const double (&v)[N])
     return row_lane<P & 3>(v[P >> 2]);
 }
 
+// The four rows' values of this lane's column (b[r] = the value row r
+// holds), by gfx950's row swaps: v_permlane16_swap exchanges rows 1 and 3
+// of one operand with rows 0 and 2 of the other, v_permlane32_swap the
+// upper half of one with the lower half of the other.  Three VALU swaps per
+// 32-bit word hand every row all four values, with no LDS traffic.  Every
+// value moves within its column, so a column whose lanes are all inactive
+// leaves the others' results untouched.
+__device__ __forceinline__ void row_spread(int v, int (&b)[4])
+{
+    const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);      // [r0 r0 r2 r2], [r1 r1 r3 r3]
+    const auto a = __builtin_amdgcn_permlane32_swap(p[0], p[0], false, false);  // [r0 r0 r0 r0], [r2 r2 r2 r2]
+    const auto c = __builtin_amdgcn_permlane32_swap(p[1], p[1], false, false);  // [r1 r1 r1 r1], [r3 r3 r3 r3]
+    b[0] = (int)a[0];
+    b[1] = (int)c[0];
+    b[2] = (int)a[1];
+    b[3] = (int)c[1];
+}
+__device__ __forceinline__ void row_spread(double v, double (&b)[4])
+{
+    int h[4], l[4];
+    row_spread(__double2hiint(v), h);
+    row_spread(__double2loint(v), l);
+#pragma unroll
+    for (int r = 0; r < 4; r++) b[r] = __hiloint2double(h[r], l[r]);
+}
+
+// Row J's value of this lane's column alone (two swaps per 32-bit word)
+template <int J>
+__device__ __forceinline__ int row_take(int v)
+{
+    static_assert(J >= 0 && J < 4, "row");
+    const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    const int h = (J & 1) ? (int)p[1] : (int)p[0];
+    const auto a = __builtin_amdgcn_permlane32_swap(h, h, false, false);
+    return (J & 2) ? (int)a[1] : (int)a[0];
+}
+template <int J>
+__device__ __forceinline__ double row_take(double v)
+{
+    return __hiloint2double(row_take<J>(__double2hiint(v)), row_take<J>(__double2loint(v)));
+}
+// plane P of the planes a column loaded with quad_load (row r holds
+// planes r, r+4, ...)
+template <int P, int N>
+__device__ __forceinline__ double col_plane(const double (&v)[N])
+{
+    static_assert(P >= 0 && (P >> 2) < N, "plane");
+    return row_take<P & 3>(v[P >> 2]);
+}
+
 // ---------------------------------------------------------------------------
 // geometry (sr_geometry.F90:147-289), once per mesh
 // ---------------------------------------------------------------------------
@@ -1221,16 +1271,19 @@ __device__ __forceinline__ void eos_staged(const Params &P, const State &s, int 
     }
 }
 
-// One cell per quad of lanes, one face per lane (sr_hllcbatten.F90:77-555):
-// each lane reconstructs, clips and solves the Riemann problem of its own
-// face; the quad then sums the four face fluxes in face order through lane
-// shuffles, so every residual is the reference's (((0+F1)+F2)+F3)+F4 without
-// atomics, and lane f finishes the conserved variables k = f, f+4, f+8.
-// Cell i's primitives and gradients are loaded once per quad (quad_load) and
-// handed to the four lanes by DPP.
+// One cell per column of lanes, one face per lane (sr_hllcbatten.F90:77-555):
+// the wave's 16 cells c hold their four faces in lanes c, c+16, c+32, c+48
+// (rows of 16 lanes), so each load instruction takes one face of 16
+// consecutive cells.  Each lane reconstructs, clips and solves the Riemann
+// problem of its own face; the column then sums the four face fluxes in face
+// order (row_spread), so every residual is the reference's
+// (((0+F1)+F2)+F3)+F4 without atomics, and the lane of face f finishes the
+// conserved variables k = f, f+4, f+8.  Cell i's primitives and gradients
+// are loaded once per column (row f loads planes f, f+4, ...) and handed to
+// the four rows by row swaps.
 template <int FLUX>
 __device__ __forceinline__ void flux_update(const Params &P, const State &s, const Mesh &m, int i, int f,
-                                            int qbase, int stage, bool store, double (&cv)[3])
+                                            int stage, bool store, double (&cv)[3])
 {
     const int ne = m.ne;
     const double *q = s.q;
@@ -1249,20 +1302,20 @@ __device__ __forceinline__ void flux_update(const Params &P, const State &s, con
         const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
         FState L, R;
         {
-            // cell i's primitives: shared by the quad
+            // cell i's primitives: shared by the column
             double v[(Q_N + 3) / 4];
             quad_load<Q_N>(q, nt, i, f, v);
-            L.rog = quad_plane<Q_ROG>(v); L.ug = quad_plane<Q_UG>(v); L.vg = quad_plane<Q_VG>(v);
-            L.wg = quad_plane<Q_WG>(v); L.tg = quad_plane<Q_TG>(v); L.phig = quad_plane<Q_PHIG>(v);
-            L.rop = quad_plane<Q_ROP>(v); L.up = quad_plane<Q_UP>(v); L.vp = quad_plane<Q_VP>(v);
-            L.wp = quad_plane<Q_WP>(v); L.tp = quad_plane<Q_TP>(v); L.pg = quad_plane<Q_PG>(v);
-            L.pp = quad_plane<Q_PP>(v);
+            L.rog = col_plane<Q_ROG>(v); L.ug = col_plane<Q_UG>(v); L.vg = col_plane<Q_VG>(v);
+            L.wg = col_plane<Q_WG>(v); L.tg = col_plane<Q_TG>(v); L.phig = col_plane<Q_PHIG>(v);
+            L.rop = col_plane<Q_ROP>(v); L.up = col_plane<Q_UP>(v); L.vp = col_plane<Q_VP>(v);
+            L.wp = col_plane<Q_WP>(v); L.tp = col_plane<Q_TP>(v); L.pg = col_plane<Q_PG>(v);
+            L.pp = col_plane<Q_PP>(v);
         }
         load_fstate(q, nt, neigh, R);
         if (P.myorder == 2) {
             // MUSCL reconstruction with one limiter per face pair
             // (sr_hllcbatten.F90:166-224), for faces with a cell on the far
-            // side.  The quad shares cell i's gradients, so the four lanes
+            // side.  The column shares cell i's gradients, so its four lanes
             // stay together here: a face with a ghost on the far side forms
             // the terms from cell i's own data and discards them.
             const bool rec = neigh < ne;
@@ -1274,9 +1327,9 @@ __device__ __forceinline__ void flux_update(const Params &P, const State &s, con
             const double xfn = xf - m.xc[nb], yfn = yf - m.yc[nb], zfn = zf - m.zc[nb];
 #define REC(fld, sl)                                                                                  \
     {                                                                                                 \
-        const double lv = L.fld + phimin * (xf0 * quad_plane<3 * (sl)>(g) +                           \
-                                            yf0 * quad_plane<3 * (sl) + 1>(g) +                       \
-                                            zf0 * quad_plane<3 * (sl) + 2>(g));                       \
+        const double lv = L.fld + phimin * (xf0 * col_plane<3 * (sl)>(g) +                            \
+                                            yf0 * col_plane<3 * (sl) + 1>(g) +                        \
+                                            zf0 * col_plane<3 * (sl) + 2>(g));                        \
         const double rv = R.fld + phimin * (xfn * GRAD(sl, 0, nb) + yfn * GRAD(sl, 1, nb) +           \
                                             zfn * GRAD(sl, 2, nb));                                   \
         if (rec) { L.fld = lv; R.fld = rv; }                                                          \
@@ -1309,12 +1362,13 @@ __device__ __forceinline__ void flux_update(const Params &P, const State &s, con
                               R.wp, R.tp, R.pp, enx, eny, enz, area, F + 5);
         }
     }
-    // lane f keeps the face sums of its own variables k = f, f+4, f+8
+    // the lane of face f keeps the face sums of its own variables k = f, f+4, f+8
     double rhs[3] = {0.0, 0.0, 0.0};
+#pragma unroll
     for (int k = 0; k < 10; k++) {
-        const double t0 = quad_get(F[k], qbase, 0), t1 = quad_get(F[k], qbase, 1),
-                     t2 = quad_get(F[k], qbase, 2), t3 = quad_get(F[k], qbase, 3);
-        const double sum = 0.0 + t0 + t1 + t2 + t3;
+        double t[4];
+        row_spread(F[k], t);
+        const double sum = 0.0 + t[0] + t[1] + t[2] + t[3];
         if ((k & 3) == f) rhs[k >> 2] = sum;
     }
     source_and_update(s, m, i, f, rhs, stage, store, cv);
@@ -1326,11 +1380,11 @@ __device__ __forceinline__ void flux_update(const Params &P, const State &s, con
 // ones) for the primitive recovery (eos_staged).  All four lanes of a quad
 // share i, so the quad takes one branch.
 template <int FLUX>
-__device__ __forceinline__ void flux_face(const Params &P, const State &s, const Mesh &m, int i, int f, int qbase,
+__device__ __forceinline__ void flux_face(const Params &P, const State &s, const Mesh &m, int i, int f,
                                           int stage, int feos, double (&cv)[3])
 {
     if (m.iblank[i] == 1) {
-        flux_update<FLUX>(P, s, m, i, f, qbase, stage, feos == 0, cv);
+        flux_update<FLUX>(P, s, m, i, f, stage, feos == 0, cv);
     } else if (feos) {
         const int ne = m.ne;
         cv[0] = s.cn[(size_t)f * ne + i];
@@ -2501,13 +2555,17 @@ __global__ void __launch_bounds__(4 * TILE) k_flux(Batch B, int stage, int feos)
     TILE_LOOP(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
-        const int l = (int)(threadIdx.x >> 2), f = (int)(threadIdx.x & 3), qbase = (int)(threadIdx.x & 63) & ~3;
+        // FP64: rows (lane c + 16 f of wave w holds face f of cell 16 w + c);
+        // FP32 bulk: quads (lanes 4 c .. 4 c + 3 of the block hold cell c)
+        const int l = F32 ? (int)(threadIdx.x >> 2) : (int)((threadIdx.x >> 6) * 16 + (threadIdx.x & 15));
+        const int f = F32 ? (int)(threadIdx.x & 3) : (int)((threadIdx.x >> 4) & 3);
+        const int qbase = (int)(threadIdx.x & 63) & ~3;
         if (l < C.count) {
             double cv[3] = {0.0, 0.0, 0.0};
             if (F32)
                 flux_face_f32<FLUX>(C.P, C.S, C.M, C.first + l, f, qbase, stage, feos, cv);
             else
-                flux_face<FLUX>(C.P, C.S, C.M, C.first + l, f, qbase, stage, feos, cv);
+                flux_face<FLUX>(C.P, C.S, C.M, C.first + l, f, stage, feos, cv);
             if (feos) {
                 stage_lds[f * TILE + l] = cv[0];
                 stage_lds[(f + 4) * TILE + l] = cv[1];
