@@ -403,13 +403,18 @@ def run_ensemble_steps(mode="graph", steps=10, warmup=2):
         s.close()
 
 
-def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
+SERVE_MAX_WAIT_S = 4.0
+
+
+def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05, max_wait_s=SERVE_MAX_WAIT_S):
     """Poisson arrivals of cases (n 10..70, 50..500 steps), admitted smallest
     work first within a 200 GB HBM budget while at most 1M active cells are
-    resident, and stepped together (FP32 bulk)."""
+    resident (a request waiting longer than max_wait_s goes first), and
+    stepped together (FP32 bulk)."""
     from hydraphase import serve as SV
     reqs = SV.poisson_requests(n_requests, rate=rate, seed=7, nmin=10, nmax=70, steps=(50, 500), precision=1)
-    srv = SV.Server(budget_bytes=200e9, max_cases=256, chunk=25, policy="shortest", fill_cells=1_000_000)
+    srv = SV.Server(budget_bytes=200e9, max_cases=256, chunk=25, policy="shortest", fill_cells=1_000_000,
+                    max_wait_s=max_wait_s)
     try:
         st = srv.run(reqs)
     finally:
@@ -424,7 +429,7 @@ def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05):
     cells = sum(int(r.mesh.iblank.sum()) for r in reqs)
     return {"workload": f"{n_requests} requests, Poisson {rate}/s, boxes n 10..70 ({cells} tets in all), "
                         "50..500 steps, 200 GB budget, smallest work first within 1M resident active cells, "
-                        "FP32 bulk",
+                        f"requests waiting over {max_wait_s:g} s first, FP32 bulk",
             "updates_per_s": st["updates_per_s"], "tts_p50_s": st["tts_p50_s"], "tts_p99_s": st["tts_p99_s"],
             "makespan_s": st["makespan_s"], "gpu_busy_s": st["gpu_busy_s"],
             "peak_resident_gb": st["peak_resident_bytes"] / 1e9, "requests_failed": st["failed"],

@@ -8,7 +8,10 @@ and the number of steps to run.  The loop
   "fifo") or smallest work first (policy "shortest": active cells x steps,
   ties by arrival), and, with a fill target, only while the resident
   active cells stay within it (a request above the target on its own runs
-  when nothing else is resident),
+  when nothing else is resident); with a wait bound (max_wait_s), a
+  request that has waited longer than the bound goes ahead of smaller
+  work, oldest first, so a stream of small requests cannot hold a large
+  one back indefinitely,
 * keeps every admitted case resident on the device,
 * advances all resident cases together, one hp_batch_step of at most
   `chunk` steps at a time (each case with its own dt), and
@@ -24,10 +27,11 @@ boundary, so their work overlaps on the device with the cases already
 there.  Every resident case advances by a chunk whatever its size, so a
 small request shares each step with the whole batch; with a fill target
 that just saturates the GPU, smallest-first admission lets short requests
-through without holding the GPU below full use.  Shortest-first can hold a
-large request back while smaller ones keep arriving; the runs here end
-their arrivals, so every request is served.  A case's arithmetic reads only its own data, so its result is the
-same bit for bit whether it ran alone or in any mix.
+through without holding the GPU below full use.  Shortest-first alone can
+hold a large request back while smaller ones keep arriving; the wait bound
+ages such a request to the front of the queue.  A case's arithmetic reads
+only its own data, so its result is the same bit for bit whether it ran
+alone or in any mix.
 
 Reported: throughput in cell-stage updates/s over the makespan, the
 time-to-solution (arrival to result on the host) p50/p99 of the requests
@@ -84,11 +88,15 @@ def poisson_requests(n: int, rate: float = 20.0, seed: int = 1, nmin: int = 10, 
 class Server:
     def __init__(self, budget_bytes: float = 200e9, max_cases: int = 256, chunk: int = 25,
                  graph: bool = False, grid=(0, 0, 0), bytes_per_cell: float = 1000.0,
-                 check_health: bool = True, policy: str = "fifo", fill_cells: int | None = None):
+                 check_health: bool = True, policy: str = "fifo", fill_cells: int | None = None,
+                 max_wait_s: float | None = None):
         if policy not in ("fifo", "shortest"):
             raise ValueError(f"unknown admission policy {policy!r} (fifo or shortest)")
+        if max_wait_s is not None and not max_wait_s >= 0.0:
+            raise ValueError(f"max_wait_s must be a non-negative number of seconds, not {max_wait_s!r}")
         self.policy = policy
         self.fill_cells = None if fill_cells is None else int(fill_cells)
+        self.max_wait_s = None if max_wait_s is None else float(max_wait_s)
         self.budget = int(budget_bytes)
         self.chunk = int(chunk)
         self.graph = graph
@@ -123,7 +131,11 @@ class Server:
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
             if self.policy == "shortest":
-                waiting = collections.deque(sorted(waiting, key=lambda r: (cells_of(r) * r.spec.steps, r.arrival)))
+                # requests past the wait bound first, oldest first; then
+                # smallest work, ties by arrival
+                aged = (lambda r: self.max_wait_s is not None and now - r.arrival > self.max_wait_s)
+                waiting = collections.deque(sorted(
+                    waiting, key=lambda r: (0, r.arrival, 0) if aged(r) else (1, cells_of(r) * r.spec.steps, r.arrival)))
             while waiting and len(active) < self.max_cases and used + waiting[0].nbytes <= self.budget:
                 if self.fill_cells is not None and active and resident + cells_of(waiting[0]) > self.fill_cells:
                     break

@@ -4,6 +4,7 @@ the HBM budget and the slot limit, smallest-work-first admission and the
 fill target, chunked stepping, retirement when done, retirement of a case
 whose health check fails, and the statistics."""
 import math
+import time
 import types
 
 import numpy as np
@@ -33,6 +34,7 @@ class FakeSolver:
 
 class FakeBatch:
     instances = []
+    step_seconds = 0.0  # wall time a step takes (for the wait-bound test)
 
     def __init__(self, max_cases, max_cells, max_ghosts):
         self.members = {}
@@ -55,6 +57,8 @@ class FakeBatch:
 
     def step(self, n, graph=True):
         self.resident_log.append(len(self.members))
+        if FakeBatch.step_seconds:
+            time.sleep(FakeBatch.step_seconds * n)
         for s in self.members.values():
             s.steps += n
 
@@ -160,3 +164,32 @@ def test_fill_target(monkeypatch):
 def test_unknown_policy_rejected():
     with pytest.raises(ValueError):
         S.Server(policy="lifo")
+
+
+def test_wait_bound_lets_a_large_request_through(monkeypatch):
+    """One slot; small requests arrive every 10 ms for 0.4 s and each takes
+    four 5 ms steps, so they queue up; a large one arrives at 15 ms.
+    Smallest-first alone serves the large one only after every small one;
+    with a 50 ms wait bound it goes ahead of the small ones waiting once it
+    has waited that long."""
+    monkeypatch.setattr(FakeBatch, "step_seconds", 0.005)
+
+    def stream():
+        reqs = requests([10] * 41, [4] * 40 + [40], cells=[1] * 40 + [100])
+        for k, r in enumerate(reqs):
+            r.arrival = 0.01 * k if k < 40 else 0.015
+        return reqs
+
+    reqs = stream()
+    run(monkeypatch, reqs, budget_bytes=1000, max_cases=1, chunk=1, policy="shortest")
+    big = reqs[40]
+    assert all(big.admit > r.admit for r in reqs[:40])  # held back behind every small request
+    reqs = stream()
+    stats, _ = run(monkeypatch, reqs, budget_bytes=1000, max_cases=1, chunk=1, policy="shortest",
+                   max_wait_s=0.05)
+    big = reqs[40]
+    assert 0.065 <= big.admit < 0.25
+    assert sum(r.arrival < big.admit < r.admit for r in reqs[:40]) >= 3  # it went ahead of waiting ones
+    assert all(r.error is None and r.result[3] == r.spec.steps for r in reqs)
+    with pytest.raises(ValueError):
+        S.Server(policy="shortest", max_wait_s=-1.0)
