@@ -137,6 +137,61 @@ __device__ __forceinline__ float quad_lane(float v)
     return __int_as_float(quad_lane32<J>(__float_as_int(v)));
 }
 
+// Lane j ^ D of this lane's quad (D = 1 or 2), a DPP quad permutation
+template <int D>
+__device__ __forceinline__ float quad_xor(float v)
+{
+    static_assert(D == 1 || D == 2, "quad xor");
+    constexpr int ctrl = D == 1 ? 0xB1 : 0x4E;  // quad_perm [1,0,3,2] / [2,3,0,1]
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float sel(bool c, float a, float b) { return c ? a : b; }
+
+// In-quad transpose of sixteen-byte items: lane j (j = lane & 3) holds
+// a[q] = item (j, q) of its quad and ends with a[c] = item (c, j), by two
+// butterfly stages (the partner lanes j ^ 1, then j ^ 2).  All four lanes
+// of the quad take part.
+__device__ __forceinline__ void quad_transpose(float4 (&a)[4], int j)
+{
+    float4 t[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const bool c = (j ^ q) & 1;
+        const float4 o = a[q ^ 1];
+        t[q] = make_float4(sel(c, quad_xor<1>(o.x), a[q].x), sel(c, quad_xor<1>(o.y), a[q].y),
+                           sel(c, quad_xor<1>(o.z), a[q].z), sel(c, quad_xor<1>(o.w), a[q].w));
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const bool c = (j ^ q) & 2;
+        const float4 o = t[q ^ 2];
+        a[q] = make_float4(sel(c, quad_xor<2>(o.x), t[q].x), sel(c, quad_xor<2>(o.y), t[q].y),
+                           sel(c, quad_xor<2>(o.z), t[q].z), sel(c, quad_xor<2>(o.w), t[q].w));
+    }
+}
+
+// The R32_N-float record of this lane's cell cq, read a quad at a time:
+// each load instruction takes four consecutive sixteen-byte pieces of one
+// of the quad's four records (one 64-byte line per quad), and in-quad
+// transposes hand every lane its own record.  All four lanes take part.
+__device__ __forceinline__ void quad_record32(const float4 *rec, int cq, int j, float (&r)[R32_N])
+{
+    int c[4];
+    c[0] = quad_lane32<0>(cq); c[1] = quad_lane32<1>(cq); c[2] = quad_lane32<2>(cq); c[3] = quad_lane32<3>(cq);
+#pragma unroll
+    for (int k = 0; k < R32_N / 16; k++) {
+        float4 a[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) a[q] = rec[(size_t)(R32_N / 4) * c[q] + 4 * k + j];
+        quad_transpose(a, j);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            r[16 * k + 4 * p] = a[p].x; r[16 * k + 4 * p + 1] = a[p].y;
+            r[16 * k + 4 * p + 2] = a[p].z; r[16 * k + 4 * p + 3] = a[p].w;
+        }
+    }
+}
+
 // Planes 0..NP-1 of one cell shared by the four lanes of its quad: lane f
 // loads planes f, f+4, f+8, ... (the last clamped to NP-1) into v, so the
 // quad issues a quarter of the loads; quad_plane<P> hands plane P to every
@@ -2008,27 +2063,41 @@ __device__ __forceinline__ FaceGeo32 face_geo32(const Mesh &m, int i, int f, int
     return g;
 }
 
+// One side's FP32 face state from its record r, reconstructed with phimin
+// and the side's face-centroid offset (xo, yo, zo) on interior faces
+__device__ __forceinline__ void side_state32(const float (&r)[R32_N], bool interior, float phimin, float xo,
+                                             float yo, float zo, FState32 &S)
+{
+#pragma clang fp contract(on)
+    rec_state32(r, S);
+    if (interior) {
+#define REC32S(fld, sl)                                                                              \
+    S.fld += phimin * (xo * r[R32_GRAD + 3 * (sl)] + yo * r[R32_GRAD + 3 * (sl) + 1] +               \
+                       zo * r[R32_GRAD + 3 * (sl) + 2]);
+        REC32S(phig, G_PHIG) REC32S(ug, G_UG) REC32S(vg, G_VG) REC32S(wg, G_WG) REC32S(tg, G_TG)
+        REC32S(pg, G_PG) REC32S(up, G_UP) REC32S(vp, G_VP) REC32S(wp, G_WP) REC32S(tp, G_TP)
+        REC32S(pp, G_PP)
+#undef REC32S
+    }
+}
+
+// the flux of a face from both sides' FP32 face states (defined below)
+template <int FLUX>
+__device__ __forceinline__ void flux_from_states32(const Params &P, const Mesh &m, int neigh, const FaceGeo32 &g,
+                                                   FState32 &L, FState32 &R, float (&F)[10]);
+
 // The FP32 flux of cell i's face towards neigh (the face loop of
 // hllcbatten / hllczien) with the face's geometry g: both sides' face states
 // (the records' primitives, reconstructed with the smaller limiter on
-// interior faces), clipping and blending, then the wall flux or the HLLC
-// flux, into F.
+// interior faces; this lane reads both records, twelve 16-byte loads each)
+// or, in first order, the primitives, then clipping and blending and the
+// wall flux or the HLLC flux, into F.
 template <int FLUX>
 __device__ __forceinline__ void face_flux_f32(const Params &P, const State &s, const Mesh &m, int i, int neigh,
                                               const FaceGeo32 &g, float (&F)[10])
 {
-#pragma clang fp contract(on)
-    const int ne = m.ne;
-    const double *q = s.q;
-    const size_t nt = s.nt;
-    const float scx = g.scx, scy = g.scy, scz = g.scz;
-    const float area = fsqr(scx * scx + scy * scy + scz * scz), ia = frcp(area);
-    const float enx = scx * ia, eny = scy * ia, enz = scz * ia;
-    const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
     FState32 L, R;
     if (P.myorder == 2) {
-        // both sides' records (cell i's and its neighbour's, a ghost's
-        // holding its primitives), twelve 16-byte loads each
         float a[R32_N], b[R32_N];
         const float4 *ra = reinterpret_cast<const float4 *>(s.rec32) + (size_t)(R32_N / 4) * i;
         const float4 *rb = reinterpret_cast<const float4 *>(s.rec32) + (size_t)(R32_N / 4) * neigh;
@@ -2038,24 +2107,29 @@ __device__ __forceinline__ void face_flux_f32(const Params &P, const State &s, c
             a[4 * j] = u.x; a[4 * j + 1] = u.y; a[4 * j + 2] = u.z; a[4 * j + 3] = u.w;
             b[4 * j] = v.x; b[4 * j + 1] = v.y; b[4 * j + 2] = v.z; b[4 * j + 3] = v.w;
         }
-        rec_state32(a, L);
-        rec_state32(b, R);
-        if (neigh < ne) {
-            const float phimin = fmn(a[R32_PHI], b[R32_PHI]);
-            const float xf0 = g.xf0, yf0 = g.yf0, zf0 = g.zf0, xfn = g.xfn, yfn = g.yfn, zfn = g.zfn;
-#define REC32(fld, sl)                                                                               \
-    L.fld += phimin * (xf0 * a[R32_GRAD + 3 * (sl)] + yf0 * a[R32_GRAD + 3 * (sl) + 1] +             \
-                       zf0 * a[R32_GRAD + 3 * (sl) + 2]);                                            \
-    R.fld += phimin * (xfn * b[R32_GRAD + 3 * (sl)] + yfn * b[R32_GRAD + 3 * (sl) + 1] +             \
-                       zfn * b[R32_GRAD + 3 * (sl) + 2]);
-            REC32(phig, G_PHIG) REC32(ug, G_UG) REC32(vg, G_VG) REC32(wg, G_WG) REC32(tg, G_TG)
-            REC32(pg, G_PG) REC32(up, G_UP) REC32(vp, G_VP) REC32(wp, G_WP) REC32(tp, G_TP) REC32(pp, G_PP)
-#undef REC32
-        }
+        const bool interior = neigh < m.ne;
+        const float phimin = fmn(a[R32_PHI], b[R32_PHI]);
+        side_state32(a, interior, phimin, g.xf0, g.yf0, g.zf0, L);
+        side_state32(b, interior, phimin, g.xfn, g.yfn, g.zfn, R);
     } else {
-        load_fstate32(q, nt, i, L);
-        load_fstate32(q, nt, neigh, R);
+        load_fstate32(s.q, s.nt, i, L);
+        load_fstate32(s.q, s.nt, neigh, R);
     }
+    flux_from_states32<FLUX>(P, m, neigh, g, L, R, F);
+}
+
+// The flux of a face from both sides' FP32 face states: clipping and
+// blending, then the wall flux (pressure terms only) or the HLLC flux.
+template <int FLUX>
+__device__ __forceinline__ void flux_from_states32(const Params &P, const Mesh &m, int neigh, const FaceGeo32 &g,
+                                                   FState32 &L, FState32 &R, float (&F)[10])
+{
+#pragma clang fp contract(on)
+    const int ne = m.ne;
+    const float scx = g.scx, scy = g.scy, scz = g.scz;
+    const float area = fsqr(scx * scx + scy * scy + scz * scz), ia = frcp(area);
+    const float enx = scx * ia, eny = scy * ia, enz = scz * ia;
+    const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
     L.phip = 1.0f - L.phig;
     R.phip = 1.0f - R.phig;
     clip_blend32(P, L);
@@ -2444,37 +2518,65 @@ __global__ void __launch_bounds__(4 * TILE) k_viscous(Batch B)
 // cells (its boundary faces and those towards higher-numbered cells; at
 // most 4 per cell), one face per lane, into the face's entry of fflux
 // (FACE32_N floats: the 10 components, the source's phig difference and
-// centroid offset), which k_flux then reads for both cells of the face
-// Held to three blocks per CU by a 48 KB static LDS reservation, as the
-// FP32 face loop of k_flux with HLLC-Zien: the compiler then keeps more
-// record loads in flight per wave (at 5.0M cells 0.82 ms against 1.18 ms
-// at 68 VGPRs, seven waves per SIMD).
-constexpr int FACE32_LDS_DOUBLES = 6 * 1024;
+// centroid offset), which k_cells32 then reads for both cells of the face.
+// A quad of lanes reads its four faces' records together (quad_record32),
+// one side at a time, each side reconstructed as soon as its record is in:
+// at 5.0M cells 0.73 ms at 117 VGPRs, four waves per SIMD, against 0.78 ms
+// with each lane reading its own faces' records (then held to three blocks
+// per CU by a 48 KB LDS reservation, under which the compiler kept more
+// record loads in flight).
 template <int MODE>
 __global__ void __launch_bounds__(4 * TILE) k_face32(Batch B)
 {
-    __shared__ double resv[FACE32_LDS_DOUBLES];
-    if (B.n1 < 0) resv[threadIdx.x] = 0.0;  // never taken (n1 >= 0): keeps the reservation
     TILE_LOOP(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
         const int base = C.S.fbase[C.first];
         const int nf = C.S.fbase[C.first + C.count] - base;
         const int l = (int)threadIdx.x;
-        if (l < nf) {
-            const int fc = C.S.flist[base + l], i = fc >> 2;
+        // a quad runs when its first face is listed; lanes past the list
+        // redo the last face (nothing stored), so a quad's record reads
+        // have all four lanes
+        if ((l & ~3) < nf) {
+            const int lf = l < nf ? l : nf - 1;
+            const int fc = C.S.flist[base + lf], i = fc >> 2;
             const int neigh = C.M.nc[(size_t)(fc & 3) * C.M.ne + i];
-            const float4 *gs = reinterpret_cast<const float4 *>(C.S.fgeom) + (size_t)3 * (base + l);
+            const float4 *gs = reinterpret_cast<const float4 *>(C.S.fgeom) + (size_t)3 * (base + lf);
             const float4 g0 = gs[0], g1 = gs[1], g2 = gs[2];
             const FaceGeo32 g = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w, g2.x, g2.y, g2.z, g2.w};
             float F[10];
-            face_flux_f32<2>(C.P, C.S, C.M, i, neigh, g, F);
-            const float dph = face_dphig32(C.S, i, neigh), dx = g.dx, dy = g.dy, dz = g.dz;
-            float4 *dst = reinterpret_cast<float4 *>(C.S.fflux) + (size_t)FACE32_N / 4 * (base + l);
-            dst[0] = make_float4(F[0], F[1], F[2], F[3]);
-            dst[1] = make_float4(F[4], F[5], F[6], F[7]);
-            dst[2] = make_float4(F[8], F[9], dph, dx);
-            dst[3] = make_float4(dy, dz, 0.0f, 0.0f);
+            if (C.P.myorder == 2) {
+                const float4 *rec = reinterpret_cast<const float4 *>(C.S.rec32);
+                const int j = l & 3;
+                const bool interior = neigh < C.M.ne;
+                // the far side's limiter first (its record's last piece), so
+                // each side is reconstructed as soon as its record is in
+                const float phib = rec[(size_t)(R32_N / 4) * neigh + (R32_PHI >> 2)].z;
+                FState32 L, R;
+                float phimin;
+                {
+                    float a[R32_N];
+                    quad_record32(rec, i, j, a);
+                    phimin = fmn(a[R32_PHI], phib);
+                    side_state32(a, interior, phimin, g.xf0, g.yf0, g.zf0, L);
+                }
+                {
+                    float b[R32_N];
+                    quad_record32(rec, neigh, j, b);
+                    side_state32(b, interior, phimin, g.xfn, g.yfn, g.zfn, R);
+                }
+                flux_from_states32<2>(C.P, C.M, neigh, g, L, R, F);
+            } else {
+                face_flux_f32<2>(C.P, C.S, C.M, i, neigh, g, F);
+            }
+            if (l < nf) {
+                const float dph = face_dphig32(C.S, i, neigh), dx = g.dx, dy = g.dy, dz = g.dz;
+                float4 *dst = reinterpret_cast<float4 *>(C.S.fflux) + (size_t)FACE32_N / 4 * (base + l);
+                dst[0] = make_float4(F[0], F[1], F[2], F[3]);
+                dst[1] = make_float4(F[4], F[5], F[6], F[7]);
+                dst[2] = make_float4(F[8], F[9], dph, dx);
+                dst[3] = make_float4(dy, dz, 0.0f, 0.0f);
+            }
         }
     }
 }
@@ -2529,8 +2631,8 @@ __global__ void __launch_bounds__(TILE) k_cells32(Batch B, int stage, int feos)
     }
 }
 
-// The FP32-bulk face loop (in k_flux with HLLC-Zien; k_face32 has its own)
-// runs fastest held to three blocks per CU by a 48 KB
+// The FP32-bulk face loop of k_flux (HLLC-Zien, each lane reading its own
+// faces' records) runs fastest held to three blocks per CU by a 48 KB
 // static LDS reservation: the compiler then schedules with more registers
 // (more gathers in flight per wave).  At 4.8M cells: 2.22 ms with it (150
 // VGPRs), 3.01 ms without (109 VGPRs, four waves per SIMD); before the
