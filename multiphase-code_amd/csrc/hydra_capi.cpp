@@ -373,7 +373,13 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     ALLOC(m.dl, ne) ALLOC(m.epssq, ne) ALLOC(m.dd, ne) ALLOC(m.fc0, 12 * (size_t)ne) ALLOC(m.fc3, 12 * (size_t)ne)
     ALLOC(s.cn, 10 * (size_t)ne) ALLOC(s.cu, 10 * (size_t)ne) ALLOC(s.cn1, 10 * (size_t)ne)
     ALLOC(s.q, (size_t)Q_N * ntot) ALLOC(s.q2, (size_t)Q_N * ntot) ALLOC(s.grad, (size_t)3 * G_N * ne) ALLOC(s.phi, ne)
-    ALLOC(s.vis, 8 * (size_t)ne) ALLOC(s.d24, 10 * (size_t)ne) ALLOC(s.d2q, 10 * (size_t)ne)
+    if (params->precision == 1) {
+        // the FP32-bulk form keeps its viscous fluxes and JST dissipation in FP32
+        ALLOC(s.visf, 8 * (size_t)ne) ALLOC(s.d24f, 10 * (size_t)ne)
+    } else {
+        ALLOC(s.vis, 8 * (size_t)ne) ALLOC(s.d24, 10 * (size_t)ne)
+    }
+    ALLOC(s.d2q, 10 * (size_t)ne)
     ALLOC(s.anu, 2 * (size_t)ne) ALLOC(s.rhs, 10 * (size_t)ne) ALLOC(s.scal, SC_N) ALLOC(s.itot, 1) ALLOC(s.mon, 4)
     ALLOC(c->health, 1)
     int *dfslot = nullptr, *dfbase = nullptr, *dflist = nullptr;
@@ -497,8 +503,13 @@ int hp_upload_state(hp_ctx *c, const double *cu, int ld, const double *gp, doubl
             HIPCHK(c, hipMemcpyAsync(qb + (size_t)slot[k] * c->s.nt + ne, gp + (size_t)k * ng, sizeof(double) * ng,
                                      hipMemcpyHostToDevice, c->st));
     // main.F90:456-476: viscous fluxes and dissipation start at zero
-    HIPCHK(c, hipMemsetAsync(c->s.vis, 0, sizeof(double) * 8 * ne, c->st));
-    HIPCHK(c, hipMemsetAsync(c->s.d24, 0, sizeof(double) * 10 * ne, c->st));
+    if (c->s.visf) {
+        HIPCHK(c, hipMemsetAsync(c->s.visf, 0, sizeof(float) * 8 * ne, c->st));
+        HIPCHK(c, hipMemsetAsync(c->s.d24f, 0, sizeof(float) * 10 * ne, c->st));
+    } else {
+        HIPCHK(c, hipMemsetAsync(c->s.vis, 0, sizeof(double) * 8 * ne, c->st));
+        HIPCHK(c, hipMemsetAsync(c->s.d24, 0, sizeof(double) * 10 * ne, c->st));
+    }
     double sc[SC_N] = {0.0, time, 0.0};
     HIPCHK(c, hipMemcpyAsync(c->s.scal, sc, sizeof(sc), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->s.itot, &total, sizeof(int), hipMemcpyHostToDevice, c->st));
@@ -780,6 +791,16 @@ int hp_probe_cell(hp_ctx *c, int cell, double *prims)
     return HP_OK;
 }
 
+// the FP32-bulk form's FP32 arrays behind array ids 4 (d24) and 10 (vis);
+// null in the FP64 form or for other ids
+static float *array_view_f32(hp_ctx *c, int which, size_t *n)
+{
+    const size_t ne = c->m.ne;
+    if (which == 4 && c->s.d24f) { *n = 10 * ne; return c->s.d24f; }
+    if (which == 10 && c->s.visf) { *n = 8 * ne; return c->s.visf; }
+    return nullptr;
+}
+
 static int array_view(hp_ctx *c, int which, double **ptr, size_t *n)
 {
     const size_t ne = c->m.ne, nt = c->s.nt;
@@ -831,6 +852,15 @@ int hp_get_array(hp_ctx *c, int which, double *host, size_t count)
     }
     double *p;
     size_t n;
+    if (float *pf = array_view_f32(c, which, &n)) {
+        // widened to FP64 on the host
+        if (count != n) return fail(c, HP_ERR_ARG, "hp_get_array: count mismatch");
+        std::vector<float> tmp(n);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), pf, sizeof(float) * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (size_t k = 0; k < n; k++) host[k] = tmp[k];
+        return HP_OK;
+    }
     if (int rc = array_view(c, which, &p, &n)) return rc;
     if (count != n) return fail(c, HP_ERR_ARG, "hp_get_array: count mismatch");
     HIPCHK(c, hipMemcpyAsync(host, p, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
@@ -844,6 +874,14 @@ int hp_set_array(hp_ctx *c, int which, const double *host, size_t count)
     if (which >= 12) return fail(c, HP_ERR_ARG, "hp_set_array: arrays 12-14 are read-only");
     double *p;
     size_t n;
+    if (float *pf = array_view_f32(c, which, &n)) {
+        // rounded to FP32 on the host
+        if (count != n) return fail(c, HP_ERR_ARG, "hp_set_array: count mismatch");
+        std::vector<float> tmp(host, host + n);
+        HIPCHK(c, hipMemcpyAsync(pf, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        return HP_OK;
+    }
     if (int rc = array_view(c, which, &p, &n)) return rc;
     if (count != n) return fail(c, HP_ERR_ARG, "hp_set_array: count mismatch");
     HIPCHK(c, hipMemcpyAsync(p, host, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
@@ -1210,7 +1248,8 @@ size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision)
     size_t b = 3 * (size_t)nodes * D;                        // coordinates
     b += (13 * ne + 4 * ng) * I;                             // nod, nc, ncp, iblank, ghost lists
     b += (12 + 3 + 12 + 12) * ne * D + 4 * nt * D;         // sc, dl/epssq/dd, fc0, fc3, xc..vol
-    b += (30 + 3 * G_N + 1 + 8 + 10 + 10 + 2 + 10) * ne * D + 2 * (size_t)Q_N * nt * D + (SC_N + 4) * D + I;
+    b += (30 + 3 * G_N + 1 + 10 + 2 + 10) * ne * D + 2 * (size_t)Q_N * nt * D + (SC_N + 4) * D + I;
+    b += (8 + 10) * ne * (precision == 1 ? sizeof(float) : D);  // viscous fluxes, JST dissipation
     if (precision == 1) {
         b += (size_t)R32_N * nt * sizeof(float);  // FP32-bulk records
         b += (9 * ne + 1) * I + 4 * ne * (FACE32_N + 12) * sizeof(float);  // face tables (HLLC-Batten; at most)

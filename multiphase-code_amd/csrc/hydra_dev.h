@@ -90,8 +90,10 @@ struct State {
     // both cells, the centroid offset between them), computed at set-up
     const int *fslot, *fbase, *flist;
     float *fflux, *fgeom;
-    double *vis;            // 8 x ne
-    double *d24, *d2q;      // 10 x ne
+    double *vis;            // 8 x ne (FP64 form; else null)
+    double *d24, *d2q;      // 10 x ne (d24: FP64 form; else null)
+    float *visf, *d24f;     // FP32-bulk form: the viscous fluxes (8 x ne) and the JST dissipation
+                            // (10 x ne), fluxes the form keeps in FP32 (else null)
     double *anu;            // 2 x ne
     double *rhs;            // 10 x ne
     const double *dl;       // ne (same as Mesh::dl)

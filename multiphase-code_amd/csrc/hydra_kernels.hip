@@ -670,7 +670,10 @@ __device__ __forceinline__ void smooth2_cell(const Params &P, const State &s, co
                         d2c[3] * (c[nb[3]] - ci);
             double d4 = d4c[0] * (w[nb[0]] - wi) + d4c[1] * (w[nb[1]] - wi) + d4c[2] * (w[nb[2]] - wi) +
                         d4c[3] * (w[nb[3]] - wi);
-            s.d24[(size_t)k * ne + i] = d2 - d4;
+            if (s.d24f)
+                s.d24f[(size_t)k * ne + i] = (float)(d2 - d4);
+            else
+                s.d24[(size_t)k * ne + i] = d2 - d4;
         }
     }
 }
@@ -1239,13 +1242,15 @@ __device__ __forceinline__ void hllc_zien(const Params &P, const FState &L, cons
 // (stage 0, per-routine checks) or applied as RK stage `stage`
 // (sr_RK.F90:29-161), whose new value is returned and, with `store`,
 // written to cn; FP64 in every precision
-__device__ __forceinline__ double rk_finish(const State &s, size_t o, double r, int stage, double dtbyvol, bool store)
+template <typename TD>
+__device__ __forceinline__ double rk_finish(const State &s, const TD *d24, size_t o, double r, int stage,
+                                            double dtbyvol, bool store)
 {
     if (stage == 0) {
         s.rhs[o] = r;
         return 0.0;
     }
-    const double rr = r - s.d24[o];
+    const double rr = r - (double)d24[o];
     const double cu = s.cu[o];
     double cn = s.cn[o];
     if (stage == 1) {
@@ -1290,7 +1295,7 @@ __device__ __forceinline__ void source_and_update(const State &s, const Mesh &m,
         case 9: r = r + src[(size_t)SRC_E * ne + i] - vis[(size_t)7 * ne + i]; break;
         default: break;
         }
-        cv[j] = rk_finish(s, (size_t)k * ne + i, r, stage, dtbyvol, store);
+        cv[j] = rk_finish(s, s.d24, (size_t)k * ne + i, r, stage, dtbyvol, store);
     }
 }
 
@@ -1456,7 +1461,7 @@ __device__ __forceinline__ void rk_cell(const State &s, const Mesh &m, int i, in
     const double dtbyvol = s.scal[SC_DT] / m.vol[i];
     for (int j = 0; j < 10; j++) {
         const size_t o = (size_t)j * ne + i;
-        const double r = s.rhs[o] - s.d24[o];
+        const double r = s.rhs[o] - (s.d24f ? (double)s.d24f[o] : s.d24[o]);
         const double cu = s.cu[o];
         double cn = s.cn[o];
         if (stage == 1) {
@@ -1925,7 +1930,7 @@ __device__ __forceinline__ double update_f32(const State &s, int ne, int i, int 
                                              double voli, int stage, double dtbyvol, bool store)
 {
 #pragma clang fp contract(on)
-    const double *vis = s.vis;
+    const float *vis = s.visf;
     switch (k) {
     case 1:
         r = r - (double)(S.fdgx + S.pint * S.dphx + S.phig * S.rog * S.grav) * voli - vis[(size_t)0 * ne + i];
@@ -1941,7 +1946,7 @@ __device__ __forceinline__ double update_f32(const State &s, int ne, int i, int 
     case 9: r = r + (double)S.work * voli - vis[(size_t)7 * ne + i]; break;
     default: break;
     }
-    return rk_finish(s, (size_t)k * ne + i, r, stage, dtbyvol, store);
+    return rk_finish(s, s.d24f, (size_t)k * ne + i, r, stage, dtbyvol, store);
 }
 
 // the quad form: lane f holds face f's offsets and finishes the conserved
@@ -2326,7 +2331,8 @@ __device__ __forceinline__ void viscous_face_f32(const Params &P, const State &s
     for (int v = 0; v < 8; v++) {
         const double t0 = (double)__shfl(acc[v], qbase, 64), t1 = (double)__shfl(acc[v], qbase + 16, 64),
                      t2 = (double)__shfl(acc[v], qbase + 32, 64), t3 = (double)__shfl(acc[v], qbase + 48, 64);
-        if ((v & 3) == f) s.vis[(size_t)v * ne + i] = 0.0 + t0 + t1 + t2 + t3;
+        // summed in FP64, kept in FP32 (the form's fluxes)
+        if ((v & 3) == f) s.visf[(size_t)v * ne + i] = (float)(0.0 + t0 + t1 + t2 + t3);
     }
 }
 
