@@ -142,6 +142,8 @@ const char *hp_last_error(const hp_ctx *ctx);
  *        12 dl (minimum edge length per cell; read-only)
  *        13, 14 the FP32-bulk form's gradients (33, order of 8) and limiter,
  *          widened to double (precision 1 only; read-only)
+ * The FP32-bulk form keeps d24 (4) and vis (10) in FP32: they are widened
+ * to double on reading and rounded to float on writing.
  * The array is variable-major with leading dimension neles (ntot for 7). */
 int hp_get_array(hp_ctx *ctx, int which, double *host, size_t count);
 int hp_set_array(hp_ctx *ctx, int which, const double *host, size_t count);
