@@ -952,12 +952,16 @@ __device__ __forceinline__ void grad_cell(const State &s, const Mesh &m, int i)
     // rog/rop (and pg/pp) feed the limiter below
     const int slot[13] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP, Q_PG, Q_PP, Q_PHIG, Q_ROG, Q_ROP};
     double glim[4][3];  // pg, pp, rog, rop
+    double dpl[4], dml[4];  // their neighbour maxima and minima less the cell's value
 #pragma unroll
     for (int sI = 0; sI < 13; sI++) {
         const int sl = slot[sI];
         const double qi = QV(sl, i);
-        double dq[4];
-        for (int k = 0; k < 4; k++) dq[k] = QV(sl, nb[k]) - qi;
+        double qn[4], dq[4];
+        for (int k = 0; k < 4; k++) {
+            qn[k] = QV(sl, nb[k]);
+            dq[k] = qn[k] - qi;
+        }
         const double c1 = divide(dq[0] * dx[0], W[0]) + divide(dq[1] * dx[1], W[1]) + divide(dq[2] * dx[2], W[2]) + divide(dq[3] * dx[3], W[3]);
         double c2;
         if (sl == Q_TP)  // sr_derivative.F90:225, last term without /wt4
@@ -974,7 +978,18 @@ __device__ __forceinline__ void grad_cell(const State &s, const Mesh &m, int i)
             s.grad[(size_t)(3 * sI + 2) * ne + i] = gz;
         }
         const int li = (sl == Q_PG) ? 0 : (sl == Q_PP) ? 1 : (sl == Q_ROG) ? 2 : (sl == Q_ROP) ? 3 : -1;
-        if (li >= 0) { glim[li][0] = gx; glim[li][1] = gy; glim[li][2] = gz; }
+        if (li >= 0) {
+            glim[li][0] = gx; glim[li][1] = gy; glim[li][2] = gz;
+            // the limiter's extrema (sr_VK.F90:89-90, dmax1/dmin1 over the
+            // cell and its neighbours in that order) from the values just read
+            double vmax = qi, vmin = qi;
+            for (int k = 0; k < 4; k++) {
+                vmax = dmax(vmax, qn[k]);
+                vmin = dmin(vmin, qn[k]);
+            }
+            dpl[li] = vmax - qi;
+            dml[li] = vmin - qi;
+        }
     }
     // limiter (sr_VK.F90:34-272)
     const double epssq = m.epssq[i];
@@ -984,23 +999,15 @@ __device__ __forceinline__ void grad_cell(const State &s, const Mesh &m, int i)
         r0[f][1] = m.fc0[(size_t)(3 * f + 1) * ne + i] - yi;
         r0[f][2] = m.fc0[(size_t)(3 * f + 2) * ne + i] - zi;
     }
-    const int lq[4] = {Q_PG, Q_PP, Q_ROG, Q_ROP};
     double ph = 0.0;
     for (int v = 0; v < 4; v++) {
-        const double qi = QV(lq[v], i);
-        double vmax = qi, vmin = qi;
-        for (int k = 0; k < 4; k++) {
-            const double qn = QV(lq[v], nb[k]);
-            vmax = dmax(vmax, qn);
-            vmin = dmin(vmin, qn);
-        }
         for (int f = 0; f < 4; f++) {
             const double delf = r0[f][0] * glim[v][0] + r0[f][1] * glim[v][1] + r0[f][2] * glim[v][2];
             double pv;
             if (delf > 0.0)
-                pv = fvenkat(vmax - qi, delf, epssq);
+                pv = fvenkat(dpl[v], delf, epssq);
             else if (delf < 0.0)
-                pv = fvenkat(vmin - qi, delf, epssq);
+                pv = fvenkat(dml[v], delf, epssq);
             else
                 pv = 1.0;
             ph = (v == 0 && f == 0) ? pv : dmin(ph, pv);
@@ -1606,14 +1613,16 @@ __device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int
     }
     const float idn = frcp(det3f(a1, b1, b2, b1, a2, b3, b2, b3, a3));
     const int slot[13] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP, Q_PG, Q_PP, Q_PHIG, Q_ROG, Q_ROP};
-    float glim[4][3];
+    float glim[4][3], dpl[4], dml[4];
 #pragma unroll
     for (int sI = 0; sI < 13; sI++) {
         const int sl = slot[sI];
         const double qi = QV(sl, i);
+        double qn[4];
+        for (int k = 0; k < 4; k++) qn[k] = QV(sl, nb[k]);
         float c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
         for (int k = 0; k < 4; k++) {
-            const float dq = (float)(QV(sl, nb[k]) - qi);
+            const float dq = (float)(qn[k] - qi);
             c1 += dq * dx[k] * iw[k];
             c2 += (sl == Q_TP && k == 3) ? dq * dy[k] : dq * dy[k] * iw[k];  // sr_derivative.F90:225
             c3 += dq * dz[k] * iw[k];
@@ -1632,7 +1641,17 @@ __device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int
                 if (4 * j + 3 <= R32_GRAD + 3 * sI + 2 && 4 * j + 3 > R32_GRAD + 3 * sI - 1) put(j);
         }
         const int li = (sl == Q_PG) ? 0 : (sl == Q_PP) ? 1 : (sl == Q_ROG) ? 2 : (sl == Q_ROP) ? 3 : -1;
-        if (li >= 0) { glim[li][0] = gx; glim[li][1] = gy; glim[li][2] = gz; }
+        if (li >= 0) {
+            glim[li][0] = gx; glim[li][1] = gy; glim[li][2] = gz;
+            // the limiter's extrema from the values just read
+            double vmax = qi, vmin = qi;
+            for (int k = 0; k < 4; k++) {
+                vmax = dmax(vmax, qn[k]);
+                vmin = dmin(vmin, qn[k]);
+            }
+            dpl[li] = (float)(vmax - qi);
+            dml[li] = (float)(vmin - qi);
+        }
     }
     const float epssq = (float)m.epssq[i];
     float r0[4][3];
@@ -1641,17 +1660,9 @@ __device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int
         r0[f][1] = (float)(m.fc0[(size_t)(3 * f + 1) * ne + i] - yi);
         r0[f][2] = (float)(m.fc0[(size_t)(3 * f + 2) * ne + i] - zi);
     }
-    const int lq[4] = {Q_PG, Q_PP, Q_ROG, Q_ROP};
     float ph = 1.0f;
     for (int v = 0; v < 4; v++) {
-        const double qi = QV(lq[v], i);
-        double vmax = qi, vmin = qi;
-        for (int k = 0; k < 4; k++) {
-            const double qn = QV(lq[v], nb[k]);
-            vmax = dmax(vmax, qn);
-            vmin = dmin(vmin, qn);
-        }
-        const float dp = (float)(vmax - qi), dm = (float)(vmin - qi);
+        const float dp = dpl[v], dm = dml[v];
         for (int f = 0; f < 4; f++) {
             const float delf = r0[f][0] * glim[v][0] + r0[f][1] * glim[v][1] + r0[f][2] * glim[v][2];
             const float pv = delf > 0.0f ? fvenkatf(dp, delf, epssq) : delf < 0.0f ? fvenkatf(dm, delf, epssq) : 1.0f;
