@@ -384,11 +384,15 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     ALLOC(c->health, 1)
     int *dfslot = nullptr, *dfbase = nullptr, *dflist = nullptr;
     if (params->precision == 1) {
-        if (params->myorder == 2) ALLOC(s.rec32, (size_t)R32_N * ntot)
+        if (params->myorder == 2) {
+            ALLOC(s.rec32, (size_t)R32_N * ntot)
+            ALLOC(s.cgeo32, (size_t)CGEO32_N * ne)
+        }
         if (!hflist.empty()) {
             ALLOC(dfslot, hfslot.size()) ALLOC(dfbase, hfbase.size()) ALLOC(dflist, hflist.size())
             ALLOC(s.fflux, (size_t)FACE32_N * hflist.size())
             ALLOC(s.fgeom, (size_t)12 * hflist.size())
+            ALLOC(s.src32, (size_t)SRC32_N * ne)
         }
     } else {
         ALLOC(s.src, (size_t)SRC_N * ne)
@@ -421,6 +425,7 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     }
     launch_geometry(m, xmax, ymax, zmax, ymin, zmin, c->st);
     launch_face32_geometry(m, s, (int)hflist.size(), c->st);
+    launch_grad32_geometry(m, s, c->st);
     e = hipStreamSynchronize(c->st);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {
@@ -1251,8 +1256,9 @@ size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision)
     b += (30 + 3 * G_N + 1 + 10 + 2 + 10) * ne * D + 2 * (size_t)Q_N * nt * D + (SC_N + 4) * D + I;
     b += (8 + 10) * ne * (precision == 1 ? sizeof(float) : D);  // viscous fluxes, JST dissipation
     if (precision == 1) {
-        b += (size_t)R32_N * nt * sizeof(float);  // FP32-bulk records
+        b += (size_t)R32_N * nt * sizeof(float) + (size_t)CGEO32_N * ne * sizeof(float);  // FP32-bulk records, geometry
         b += (9 * ne + 1) * I + 4 * ne * (FACE32_N + 12) * sizeof(float);  // face tables (HLLC-Batten; at most)
+        b += SRC32_N * ne * sizeof(float);                                   // its source terms
     }
     else b += SRC_N * ne * D;                                     // interphase source terms
     const size_t tc = (ne + TILE - 1) / TILE, tg = (ng + TILE - 1) / TILE + 1;

@@ -30,8 +30,14 @@ enum { SC_DT, SC_TIME, SC_DTMIN, SC_N };
 // primitives only; a cell that is never updated keeps zero gradients and
 // limiter, as the reference keeps its derivative arrays.
 enum { R32_PRIM = 0, R32_GRAD = 13, R32_PHI = 46, R32_N = 48 };
-// floats per face entry of State::fflux
-enum { FACE32_N = 16 };
+// floats per face entry of State::fflux: the 10 flux components, 2 pad
+enum { FACE32_N = 12 };
+// the FP32-bulk HLLC-Batten form's interphase source, per cell, as the five
+// FP32 terms the stage update widens and multiplies by the volume (variable
+// 1, variables 2 and 7, 3 and 8, 4 and 9, variable 6)
+enum { SRC32_N = 5 };
+// planes of State::cgeo32
+enum { CGEO32_N = 24 };
 
 // CFL bound of one cell from its primitives (main.F90:525-546): dl over the
 // larger phase sound speed plus the larger phase speed.  q is variable-major
@@ -82,14 +88,18 @@ struct State {
     // face, or one towards a higher-numbered cell) are listed cell by cell:
     // flist[k] = 4*cell + face, cell i's from fbase[i] to fbase[i+1] (ne+1
     // entries); fflux holds FACE32_N floats per listed face (10 flux
-    // components, the phig difference and centroid offset x, y, z towards
-    // the other cell, 2 pad); fslot (4 x ne, face-major) gives each cell
+    // components, 2 pad); fslot (4 x ne, face-major) gives each cell
     // face its entry as (k << 1) | 1 when the cell takes the listed values
     // with the opposite sign; fgeom holds each listed face's FP32 geometry,
     // 12 floats (FaceGeo32 order: area vector, face-centroid offsets from
     // both cells, the centroid offset between them), computed at set-up
     const int *fslot, *fbase, *flist;
     float *fflux, *fgeom;
+    float *src32;           // FP32-bulk HLLC-Batten: SRC32_N x ne source terms, written by the gradient
+                            // pass (else null)
+    float *cgeo32;          // FP32-bulk second order: CGEO32_N x ne, each cell's neighbour centroid offsets
+                            // (x of faces 0..3, then y, z) and face-centroid offsets (x, y, z), rounded
+                            // from FP64 once at set-up (else null)
     double *vis;            // 8 x ne (FP64 form; else null)
     double *d24, *d2q;      // 10 x ne (d24: FP64 form; else null)
     float *visf, *d24f;     // FP32-bulk form: the viscous fluxes (8 x ne) and the JST dissipation
@@ -151,6 +161,7 @@ struct Grid {
 void launch_geometry(const Mesh &m, double xmax, double ymax, double zmax, double ymin, double zmin,
                      hipStream_t st);
 void launch_face32_geometry(const Mesh &m, const State &s, int nf, hipStream_t st);
+void launch_grad32_geometry(const Mesh &m, const State &s, hipStream_t st);
 void launch_eos(const Batch &B, const Grid &G, int want_dt, hipStream_t st);
 void launch_dt_final(const Batch &B, const Grid &G, hipStream_t st);
 void launch_bc(const Batch &B, const Grid &G, int viscous, hipStream_t st);

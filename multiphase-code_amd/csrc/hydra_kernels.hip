@@ -1561,10 +1561,16 @@ __device__ __forceinline__ float fvenkatf(float delp, float delm, float epssq)
     return num * frcp(den * delm);
 }
 
+__device__ __forceinline__ void cell_source32(const Params &P, const State &s, const Mesh &m, int i,
+                                              const float (&dx)[4], const float (&dy)[4], const float (&dz)[4],
+                                              const float (&dph)[4], float (&T)[SRC32_N]);
+
 // gradients (sr_derivative.F90:45-322, incl. the cTp2 defect) and limiter
 // (sr_VK.F90:32-274) in FP32, one thread per cell, written with the cell's
-// primitives as its record (R32_*)
-__device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int i, float *lrec)
+// primitives as its record (R32_*); with src32 (HLLC-Batten) also the
+// cell's interphase source terms (sr_source.F90:46-155), from the centroid
+// offsets and phig differences the gradients use
+__device__ __forceinline__ void grad_cell_f32(const Params &P, const State &s, const Mesh &m, int i, float *lrec)
 {
 #pragma clang fp contract(on)
     const int ne = m.ne;
@@ -1594,12 +1600,13 @@ __device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int
     }
     int nb[4];
     for (int k = 0; k < 4; k++) nb[k] = m.nc[(size_t)k * ne + i];
-    const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i];
+    // the neighbours' centroid offsets, rounded at set-up (grad32_geo)
+    const float *cg = s.cgeo32;
     float dx[4], dy[4], dz[4], iw[4];
     for (int k = 0; k < 4; k++) {
-        dx[k] = (float)(m.xc[nb[k]] - xi);
-        dy[k] = (float)(m.yc[nb[k]] - yi);
-        dz[k] = (float)(m.zc[nb[k]] - zi);
+        dx[k] = cg[(size_t)k * ne + i];
+        dy[k] = cg[(size_t)(4 + k) * ne + i];
+        dz[k] = cg[(size_t)(8 + k) * ne + i];
         iw[k] = frcp(dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k]);
     }
     float a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
@@ -1610,6 +1617,16 @@ __device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int
         b1 += dx[k] * dy[k] * iw[k];
         b2 += dx[k] * dz[k] * iw[k];
         b3 += dz[k] * dy[k] * iw[k];
+    }
+    if (s.src32) {
+        // the source first, while the cell's FP64 primitives are in
+        // registers; its phig differences are the gradient's (slot Q_PHIG)
+        float dph[4], T[SRC32_N];
+        const double phi_i = QV(Q_PHIG, i);
+        for (int k = 0; k < 4; k++) dph[k] = (float)(QV(Q_PHIG, nb[k]) - phi_i);
+        cell_source32(P, s, m, i, dx, dy, dz, dph, T);
+#pragma unroll
+        for (int j = 0; j < SRC32_N; j++) s.src32[(size_t)j * ne + i] = T[j];
     }
     const float idn = frcp(det3f(a1, b1, b2, b1, a2, b3, b2, b3, a3));
     const int slot[13] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP, Q_PG, Q_PP, Q_PHIG, Q_ROG, Q_ROP};
@@ -1656,9 +1673,9 @@ __device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int
     const float epssq = (float)m.epssq[i];
     float r0[4][3];
     for (int f = 0; f < 4; f++) {
-        r0[f][0] = (float)(m.fc0[(size_t)(3 * f) * ne + i] - xi);
-        r0[f][1] = (float)(m.fc0[(size_t)(3 * f + 1) * ne + i] - yi);
-        r0[f][2] = (float)(m.fc0[(size_t)(3 * f + 2) * ne + i] - zi);
+        r0[f][0] = cg[(size_t)(12 + f) * ne + i];
+        r0[f][1] = cg[(size_t)(16 + f) * ne + i];
+        r0[f][2] = cg[(size_t)(20 + f) * ne + i];
     }
     float ph = 1.0f;
     for (int v = 0; v < 4; v++) {
@@ -1671,6 +1688,28 @@ __device__ __forceinline__ void grad_cell_f32(const State &s, const Mesh &m, int
     }
     r[R32_PHI] = ph;
     put(R32_N / 4 - 1);
+}
+
+// the source terms alone (first-order FP32-bulk HLLC-Batten)
+__device__ __forceinline__ void source_cell_f32(const Params &P, const State &s, const Mesh &m, int i)
+{
+    if (m.iblank[i] != 1) return;
+    const int ne = m.ne;
+    const double *q = s.q;
+    const size_t nt = s.nt;
+    const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i], phi_i = QV(Q_PHIG, i);
+    float dx[4], dy[4], dz[4], dph[4];
+    for (int k = 0; k < 4; k++) {
+        const int nb = m.nc[(size_t)k * ne + i];
+        dx[k] = (float)(m.xc[nb] - xi);
+        dy[k] = (float)(m.yc[nb] - yi);
+        dz[k] = (float)(m.zc[nb] - zi);
+        dph[k] = (float)(QV(Q_PHIG, nb) - phi_i);
+    }
+    float T[SRC32_N];
+    cell_source32(P, s, m, i, dx, dy, dz, dph, T);
+#pragma unroll
+    for (int j = 0; j < SRC32_N; j++) s.src32[(size_t)j * ne + i] = T[j];
 }
 
 struct FState32 {
@@ -1935,26 +1974,75 @@ __device__ __forceinline__ Src32 source_f32(const Params &P, const State &s, con
     return r;
 }
 
+// The five FP32 terms the stage update takes from Src32 (SRC32_N order:
+// variable 1, variables 2 and 7, 3 and 8, 4 and 9, variable 6), each one
+// expression, so every form that computes them rounds alike
+__device__ __forceinline__ void src_terms32(const Src32 &S, float (&T)[SRC32_N])
+{
+#pragma clang fp contract(on)
+    T[0] = S.fdgx + S.pint * S.dphx + S.phig * S.rog * S.grav;
+    T[1] = S.fdgy + S.pint * S.dphy;
+    T[2] = S.fdgz + S.pint * S.dphz;
+    T[3] = S.work;
+    T[4] = S.fdgx + S.pint * S.dphx - S.phip * S.rop * S.grav;
+}
+
+// the source's 1/|d| least-squares sums over a cell's four faces
+// (sr_source.F90:60-95) from the faces' centroid offsets and phig
+// differences: each face's terms as their own statements (no contraction
+// across faces), then the sums in face order
+__device__ __forceinline__ void src_lsq32(const float (&dx)[4], const float (&dy)[4], const float (&dz)[4],
+                                          const float (&dph)[4], float (&sum)[9])
+{
+#pragma clang fp contract(on)
+    float t[9][4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const float iw = frcp(fsqr(dx[f] * dx[f] + dy[f] * dy[f] + dz[f] * dz[f]));
+        t[0][f] = dx[f] * dx[f] * iw;
+        t[1][f] = dy[f] * dy[f] * iw;
+        t[2][f] = dz[f] * dz[f] * iw;
+        t[3][f] = dx[f] * dy[f] * iw;
+        t[4][f] = dx[f] * dz[f] * iw;
+        t[5][f] = dy[f] * dz[f] * iw;
+        t[6][f] = dph[f] * dx[f] * iw;
+        t[7][f] = dph[f] * dy[f] * iw;
+        t[8][f] = dph[f] * dz[f] * iw;
+    }
+#pragma unroll
+    for (int j = 0; j < 9; j++) sum[j] = t[j][0] + t[j][1] + t[j][2] + t[j][3];
+}
+
+// cell i's source terms (SRC32_N) from its four faces' offsets and phig
+// differences
+__device__ __forceinline__ void cell_source32(const Params &P, const State &s, const Mesh &m, int i,
+                                              const float (&dx)[4], const float (&dy)[4], const float (&dz)[4],
+                                              const float (&dph)[4], float (&T)[SRC32_N])
+{
+    float sum[9];
+    src_lsq32(dx, dy, dz, dph, sum);
+    const Src32 S = source_f32(P, s, m, i, sum[0], sum[1], sum[2], sum[3], sum[4], sum[5], sum[6], sum[7], sum[8]);
+    src_terms32(S, T);
+}
+
 // the stage update of conserved variable k of cell i from its face sum r:
-// the source term and viscous sum joined in FP64, then the RK stage
-__device__ __forceinline__ double update_f32(const State &s, int ne, int i, int k, double r, const Src32 &S,
-                                             double voli, int stage, double dtbyvol, bool store)
+// the source term (T, SRC32_N order) and viscous sum joined in FP64, then
+// the RK stage
+__device__ __forceinline__ double update_f32(const State &s, int ne, int i, int k, double r,
+                                             const float (&T)[SRC32_N], double voli, int stage, double dtbyvol,
+                                             bool store)
 {
 #pragma clang fp contract(on)
     const float *vis = s.visf;
     switch (k) {
-    case 1:
-        r = r - (double)(S.fdgx + S.pint * S.dphx + S.phig * S.rog * S.grav) * voli - vis[(size_t)0 * ne + i];
-        break;
-    case 2: r = r - (double)(S.fdgy + S.pint * S.dphy) * voli - vis[(size_t)1 * ne + i]; break;
-    case 3: r = r - (double)(S.fdgz + S.pint * S.dphz) * voli - vis[(size_t)2 * ne + i]; break;
-    case 4: r = r - (double)S.work * voli - vis[(size_t)3 * ne + i]; break;
-    case 6:
-        r = r + (double)(S.fdgx + S.pint * S.dphx - S.phip * S.rop * S.grav) * voli - vis[(size_t)4 * ne + i];
-        break;
-    case 7: r = r + (double)(S.fdgy + S.pint * S.dphy) * voli - vis[(size_t)5 * ne + i]; break;
-    case 8: r = r + (double)(S.fdgz + S.pint * S.dphz) * voli - vis[(size_t)6 * ne + i]; break;
-    case 9: r = r + (double)S.work * voli - vis[(size_t)7 * ne + i]; break;
+    case 1: r = r - (double)T[0] * voli - vis[(size_t)0 * ne + i]; break;
+    case 2: r = r - (double)T[1] * voli - vis[(size_t)1 * ne + i]; break;
+    case 3: r = r - (double)T[2] * voli - vis[(size_t)2 * ne + i]; break;
+    case 4: r = r - (double)T[3] * voli - vis[(size_t)3 * ne + i]; break;
+    case 6: r = r + (double)T[4] * voli - vis[(size_t)4 * ne + i]; break;
+    case 7: r = r + (double)T[1] * voli - vis[(size_t)5 * ne + i]; break;
+    case 8: r = r + (double)T[2] * voli - vis[(size_t)6 * ne + i]; break;
+    case 9: r = r + (double)T[3] * voli - vis[(size_t)7 * ne + i]; break;
     default: break;
     }
     return rk_finish(s, s.d24f, (size_t)k * ne + i, r, stage, dtbyvol, store);
@@ -1981,62 +2069,47 @@ __device__ __forceinline__ void source_and_update_f32(const Params &P, const Sta
         c3 = quad_sum4f(dph * dz * iw, qbase);
     }
     const Src32 S = source_f32(P, s, m, i, a1, a2, a3, b1, b2, b3, c1, c2, c3);
+    float T[SRC32_N];
+    src_terms32(S, T);
     const double voli = m.vol[i];
     const double dtbyvol = (stage > 0) ? s.scal[SC_DT] / voli : 0.0;
     for (int j = 0; j < 3; j++) {
         const int k = f + 4 * j;
         if (k >= 10) break;
-        cv[j] = update_f32(s, m.ne, i, k, rhs[j], S, voli, stage, dtbyvol, store);
+        cv[j] = update_f32(s, m.ne, i, k, rhs[j], T, voli, stage, dtbyvol, store);
     }
 }
 
 // the cell form (HLLC-Batten, after k_face32): one thread per cell reads its
-// four face entries, sums the fluxes in face order (widened to FP64) and
-// the source's LSQ terms in face order, and finishes all ten conserved
-// variables
-__device__ __forceinline__ void cell_update_f32(const Params &P, const State &s, const Mesh &m, int i, int stage,
-                                                bool store, double (&cv)[10])
+// four face entries and sums the fluxes in face order (widened to FP64),
+// takes its source terms from the gradient pass (src32) and finishes all
+// ten conserved variables
+__device__ __forceinline__ void cell_update_f32(const State &s, const Mesh &m, int i, int stage, bool store,
+                                                double (&cv)[10])
 {
 #pragma clang fp contract(on)
     const int ne = m.ne;
-    float F[4][10], dx[4], dy[4], dz[4], dph[4];
+    float F[4][10];
 #pragma unroll
     for (int f = 0; f < 4; f++) {
         const int code = s.fslot[(size_t)f * ne + i];
         const float4 *src = reinterpret_cast<const float4 *>(s.fflux) + (size_t)FACE32_N / 4 * (code >> 1);
-        const float4 u0 = src[0], u1 = src[1], u2 = src[2], u3 = src[3];
+        const float4 u0 = src[0], u1 = src[1];
+        const float2 u2 = *reinterpret_cast<const float2 *>(src + 2);
         const float sg = (code & 1) ? -1.0f : 1.0f;
         F[f][0] = sg * u0.x; F[f][1] = sg * u0.y; F[f][2] = sg * u0.z; F[f][3] = sg * u0.w;
         F[f][4] = sg * u1.x; F[f][5] = sg * u1.y; F[f][6] = sg * u1.z; F[f][7] = sg * u1.w;
         F[f][8] = sg * u2.x; F[f][9] = sg * u2.y;
-        dph[f] = sg * u2.z; dx[f] = sg * u2.w; dy[f] = sg * u3.x; dz[f] = sg * u3.y;
     }
-    // each face's LSQ terms as their own statements (no contraction across
-    // faces), then the sums in face order
-    float t[9][4];
+    float T[SRC32_N];
 #pragma unroll
-    for (int f = 0; f < 4; f++) {
-        const float iw = frcp(fsqr(dx[f] * dx[f] + dy[f] * dy[f] + dz[f] * dz[f]));
-        t[0][f] = dx[f] * dx[f] * iw;
-        t[1][f] = dy[f] * dy[f] * iw;
-        t[2][f] = dz[f] * dz[f] * iw;
-        t[3][f] = dx[f] * dy[f] * iw;
-        t[4][f] = dx[f] * dz[f] * iw;
-        t[5][f] = dy[f] * dz[f] * iw;
-        t[6][f] = dph[f] * dx[f] * iw;
-        t[7][f] = dph[f] * dy[f] * iw;
-        t[8][f] = dph[f] * dz[f] * iw;
-    }
-    float sum[9];
-#pragma unroll
-    for (int j = 0; j < 9; j++) sum[j] = t[j][0] + t[j][1] + t[j][2] + t[j][3];
-    const Src32 S = source_f32(P, s, m, i, sum[0], sum[1], sum[2], sum[3], sum[4], sum[5], sum[6], sum[7], sum[8]);
+    for (int j = 0; j < SRC32_N; j++) T[j] = s.src32[(size_t)j * ne + i];
     const double voli = m.vol[i];
     const double dtbyvol = (stage > 0) ? s.scal[SC_DT] / voli : 0.0;
 #pragma unroll
     for (int k = 0; k < 10; k++) {
         const double r = 0.0 + (double)F[0][k] + (double)F[1][k] + (double)F[2][k] + (double)F[3][k];
-        cv[k] = update_f32(s, ne, i, k, r, S, voli, stage, dtbyvol, store);
+        cv[k] = update_f32(s, ne, i, k, r, T, voli, stage, dtbyvol, store);
     }
 }
 
@@ -2486,28 +2559,46 @@ __global__ void __launch_bounds__(TILE) k_smooth2(Batch B)
 // second the tile's interphase source terms, so that neither path's
 // registers add to the other's; without GRAD (first order) one wave computes
 // the source.
-template <int MODE, bool F32, bool GRAD>
-__global__ void __launch_bounds__((!F32 && GRAD) ? 2 * TILE : TILE) k_grad_limiter(Batch B)
+template <int MODE, bool GRAD>
+__global__ void __launch_bounds__(GRAD ? 2 * TILE : TILE) k_grad_limiter(Batch B)
 {
     TILE_LOOP(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
         const int lane = (int)(threadIdx.x & 63u);
-        if (F32) {
-            // records staged in LDS, then stored as one contiguous run of
-            // the tile's count x R32_N floats (consecutive cells' records
-            // are adjacent)
-            __shared__ float4 stage32[TILE * R32_N / 4];
-            if (lane < C.count) grad_cell_f32(C.S, C.M, C.first + lane, reinterpret_cast<float *>(stage32) + R32_N * lane);
-            __syncthreads();
-            float4 *out = reinterpret_cast<float4 *>(C.S.rec32) + (size_t)(R32_N / 4) * C.first;
-            for (int k = lane; k < C.count * (R32_N / 4); k += TILE) out[k] = stage32[k];
-            if (MODE == 1) __syncthreads();  // the next tile reuses the staging
-        } else if (lane < C.count) {
+        if (lane < C.count) {
             if (GRAD && threadIdx.x < TILE)
                 grad_cell(C.S, C.M, C.first + lane);
             else
                 source_cell(C.P, C.S, C.M, C.first + lane);
+        }
+    }
+}
+
+// The FP32-bulk form's gradient pass: records, and HLLC-Batten's source
+// terms (without GRAD, first order: the source terms alone).  The
+// single-case launch form takes 134 VGPRs (three waves per SIMD); held to
+// four waves (128 VGPRs) it runs no faster on the 5.2M-tet case.
+template <int MODE, bool GRAD>
+__global__ void __launch_bounds__(TILE) k_grad32(Batch B)
+{
+    TILE_LOOP(B.cnt[0], B.n1)
+    {
+        const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
+        const int lane = (int)threadIdx.x;
+        if (!GRAD) {
+            if (lane < C.count) source_cell_f32(C.P, C.S, C.M, C.first + lane);
+        } else {
+            // records staged in LDS, then stored as one contiguous run of
+            // the tile's count x R32_N floats (consecutive cells' records
+            // are adjacent)
+            __shared__ float4 stage32[TILE * R32_N / 4];
+            if (lane < C.count)
+                grad_cell_f32(C.P, C.S, C.M, C.first + lane, reinterpret_cast<float *>(stage32) + R32_N * lane);
+            __syncthreads();
+            float4 *out = reinterpret_cast<float4 *>(C.S.rec32) + (size_t)(R32_N / 4) * C.first;
+            for (int k = lane; k < C.count * (R32_N / 4); k += TILE) out[k] = stage32[k];
+            if (MODE == 1) __syncthreads();  // the next tile reuses the staging
         }
     }
 }
@@ -2534,8 +2625,8 @@ __global__ void __launch_bounds__(4 * TILE) k_viscous(Batch B)
 // FP32-bulk HLLC-Batten: the flux of every face listed for the tile's
 // cells (its boundary faces and those towards higher-numbered cells; at
 // most 4 per cell), one face per lane, into the face's entry of fflux
-// (FACE32_N floats: the 10 components, the source's phig difference and
-// centroid offset), which k_cells32 then reads for both cells of the face.
+// (FACE32_N floats: the 10 components, 2 pad), which k_cells32 then reads
+// for both cells of the face.
 // A quad of lanes reads its four faces' records together (quad_record32),
 // one side at a time, each side reconstructed as soon as its record is in:
 // at 5.0M cells 0.73 ms at 117 VGPRs, four waves per SIMD, against 0.78 ms
@@ -2587,12 +2678,10 @@ __global__ void __launch_bounds__(4 * TILE) k_face32(Batch B)
                 face_flux_f32<2>(C.P, C.S, C.M, i, neigh, g, F);
             }
             if (l < nf) {
-                const float dph = face_dphig32(C.S, i, neigh), dx = g.dx, dy = g.dy, dz = g.dz;
                 float4 *dst = reinterpret_cast<float4 *>(C.S.fflux) + (size_t)FACE32_N / 4 * (base + l);
                 dst[0] = make_float4(F[0], F[1], F[2], F[3]);
                 dst[1] = make_float4(F[4], F[5], F[6], F[7]);
-                dst[2] = make_float4(F[8], F[9], dph, dx);
-                dst[3] = make_float4(dy, dz, 0.0f, 0.0f);
+                dst[2] = make_float4(F[8], F[9], 0.0f, 0.0f);
             }
         }
     }
@@ -2616,8 +2705,34 @@ void launch_face32_geometry(const Mesh &m, const State &s, int nf, hipStream_t s
     if (nf > 0) hipLaunchKernelGGL(k_face32_geom, dim3(nblk(nf, 256)), dim3(256), 0, st, m, s, nf);
 }
 
+// the FP32-bulk gradient pass's geometry (State::cgeo32), once: each
+// neighbour's centroid offset and each face centroid's offset from the
+// cell's centroid, differences formed in FP64 and rounded
+__global__ void k_grad32_geom(Mesh m, State s)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, ne = m.ne;
+    if (i >= ne) return;
+    const double xi = m.xc[i], yi = m.yc[i], zi = m.zc[i];
+    float *cg = s.cgeo32;
+    for (int k = 0; k < 4; k++) {
+        const int nb = m.nc[(size_t)k * ne + i];
+        cg[(size_t)k * ne + i] = (float)(m.xc[nb] - xi);
+        cg[(size_t)(4 + k) * ne + i] = (float)(m.yc[nb] - yi);
+        cg[(size_t)(8 + k) * ne + i] = (float)(m.zc[nb] - zi);
+        cg[(size_t)(12 + k) * ne + i] = (float)(m.fc0[(size_t)(3 * k) * ne + i] - xi);
+        cg[(size_t)(16 + k) * ne + i] = (float)(m.fc0[(size_t)(3 * k + 1) * ne + i] - yi);
+        cg[(size_t)(20 + k) * ne + i] = (float)(m.fc0[(size_t)(3 * k + 2) * ne + i] - zi);
+    }
+}
+
+void launch_grad32_geometry(const Mesh &m, const State &s, hipStream_t st)
+{
+    if (s.cgeo32 && m.ne > 0) hipLaunchKernelGGL(k_grad32_geom, dim3(nblk(m.ne, 256)), dim3(256), 0, st, m, s);
+}
+
 // FP32-bulk HLLC-Batten after k_face32: one thread per cell sums its face
-// entries, adds the source and runs the stage update (cell_update_f32) and,
+// entries, adds the source terms the gradient pass stored (src32) and runs
+// the stage update (cell_update_f32) and,
 // in the fused step (feos), the cell's primitive recovery (eos_staged), as
 // k_flux does for the four-lanes-per-cell forms
 template <int MODE>
@@ -2632,7 +2747,7 @@ __global__ void __launch_bounds__(TILE) k_cells32(Batch B, int stage, int feos)
             const int i = C.first + l, ne = C.M.ne;
             double cv[10];
             if (C.M.iblank[i] == 1) {
-                cell_update_f32(C.P, C.S, C.M, i, stage, feos == 0, cv);
+                cell_update_f32(C.S, C.M, i, stage, feos == 0, cv);
             } else if (feos) {
                 for (int k = 0; k < 10; k++) cv[k] = C.S.cn[(size_t)k * ne + i];
             }
@@ -2810,13 +2925,17 @@ void launch_viscous(const Batch &B, const Grid &G, hipStream_t st)
 
 void launch_grad(const Batch &B, const Grid &G, hipStream_t st)
 {
-    if (G.precision == 1) {  // the FP32-bulk flux forms its source itself
+    if (G.precision == 1) {
+        // HLLC-Batten's source terms come from this pass (with the gradients
+        // in second order); the HLLC-Zien flux forms its source itself
         if (G.myorder == 2)
-            hipLaunchKernelGGL(PICK(1, k_grad_limiter, true, true), dim3(G.cell), dim3(TILE), 0, st, B);
+            hipLaunchKernelGGL(PICK(1, k_grad32, true), dim3(G.cell), dim3(TILE), 0, st, B);
+        else if (G.myflux == 2)
+            hipLaunchKernelGGL(PICK(1, k_grad32, false), dim3(G.cell), dim3(TILE), 0, st, B);
     } else if (G.myorder == 2) {
-        hipLaunchKernelGGL(PICK(1, k_grad_limiter, false, true), dim3(G.cell), dim3(2 * TILE), 0, st, B);
+        hipLaunchKernelGGL(PICK(1, k_grad_limiter, true), dim3(G.cell), dim3(2 * TILE), 0, st, B);
     } else {
-        hipLaunchKernelGGL(PICK(1, k_grad_limiter, false, false), dim3(G.cell), dim3(TILE), 0, st, B);
+        hipLaunchKernelGGL(PICK(1, k_grad_limiter, false), dim3(G.cell), dim3(TILE), 0, st, B);
     }
 }
 
