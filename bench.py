@@ -327,7 +327,7 @@ def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
     t0 = time.time()
     tuning = HB.tune(specs, precision=1, log=log)
     log(f"# tuned in {time.time() - t0:.1f} s: morton={tuning['morton']} slabs={tuning['slabs']} "
-        f"grid={tuning['grid']}")
+        f"grid={tuning['grid']} block={tuning['block']}")
     res = HB.run_ensemble(specs, precision=1, tuning=tuning, graph=True)
     batched = res["updates"] / (res["ms"] * 1e-3)
     # plain one-case FP64 path: each case alone (in the same cell order),
@@ -367,7 +367,8 @@ def ensemble_bench(log, steps=200, sample=3, baseline_steps=20):
                         f"R 0.05-0.25, half viscous, {steps} steps each",
             "updates_per_s": batched, "ms_per_step": res["ms"] / steps,
             "plain_fp64_one_case_updates_per_s": plain, "speedup_vs_plain_fp64": batched / plain,
-            "tuning": {"morton": tuning["morton"], "slabs": tuning["slabs"], "grid": list(tuning["grid"])},
+            "tuning": {"morton": tuning["morton"], "slabs": tuning["slabs"], "grid": list(tuning["grid"]),
+                       "block": tuning["block"]},
             "mixture_rel_l2_vs_fp64_sample": errs,
             "mixture_rel_l2_floor_sample": floors, "sample_cases": sample,
             "cases_unhealthy_at_end": int(sum(1 for h in res["health"] if h))}

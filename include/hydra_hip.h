@@ -199,6 +199,12 @@ int hp_batch_remove(hp_batch *b, hp_ctx *ctx);
  * graph replays (the graph outlives membership changes), per tile in use
  * for eager steps.  Chosen by the tuner in hydraphase.batch. */
 int hp_batch_set_grid(hp_batch *b, int cell_blocks, int quad_blocks, int ghost_blocks);
+/* block size of the streaming per-cell kernels (EOS, smoothing, the FP32
+ * cell pass, stage update, end of step): tiles_per_block 1, 2 or 4 tiles
+ * of 64 cells, one per wave, i.e. blocks of 64, 128 or 256 threads (their
+ * block count follows cell_blocks / tiles_per_block).  Chosen by the tuner
+ * in hydraphase.batch; results do not depend on it. */
+int hp_batch_set_block(hp_batch *b, int tiles_per_block);
 /* nsteps time steps of every member; use_graph: replay one captured step */
 int hp_batch_step(hp_batch *b, int nsteps, int use_graph);
 int hp_batch_synchronize(hp_batch *b);

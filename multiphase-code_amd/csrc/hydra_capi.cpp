@@ -64,6 +64,7 @@ struct hp_batch {
     std::vector<hp_ctx *> members;  // per slot, null when free
     int ncases = 0;
     int grid_cell = 0, grid_quad = 0, grid_ghost = 0;  // 0: one block per tile at capacity
+    int tiles_per_block = 1;                            // streaming per-cell kernels (hp_batch_set_block)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -452,6 +453,8 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
     c->G.cell = round8(c->ts.nct);
     c->G.quad = round8(c->ts.nct);
     c->G.ghost = round8(c->ts.ngt);
+    c->G.tpb = 1;
+    c->G.cellw = c->G.cell;
     c->G.slots = 1;
     c->G.persist = 0;
     c->G.single = 1;
@@ -1002,6 +1005,15 @@ static int bfail(hp_batch *b, int code, const std::string &msg)
             return bfail((b), HP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
     } while (0)
 
+// blocks of G.tpb tiles for the streaming per-cell kernels: as many as the
+// cell family's one-tile blocks hold tiles, rounded up to multiples of 8;
+// they loop over tiles (persist bit 8) when they hold fewer than ntiles
+static void cellw_grid(Grid &G, int ntiles)
+{
+    G.cellw = round8((G.cell + G.tpb - 1) / G.tpb);
+    G.persist = (G.persist & ~8) | ((long long)G.cellw * G.tpb < ntiles ? 8 : 0);
+}
+
 static void batch_grids(hp_batch *b)
 {
     b->G.cell = b->grid_cell ? round8(b->grid_cell) : round8(b->ts.ct_cap);
@@ -1009,9 +1021,11 @@ static void batch_grids(hp_batch *b)
     b->G.ghost = b->grid_ghost ? round8(b->grid_ghost) : round8(b->ts.gt_cap);
     b->G.slots = b->ts.slots;
     b->G.single = 0;
+    b->G.tpb = b->tiles_per_block;
     // a grid with fewer blocks than the tables can hold tiles loops
     b->G.persist = (b->G.cell < b->ts.ct_cap ? 1 : 0) | (b->G.quad < b->ts.ct_cap ? 2 : 0) |
                    (b->G.ghost < b->ts.gt_cap ? 4 : 0);
+    cellw_grid(b->G, b->ts.ct_cap);
 }
 
 // forget the captured step; a launch of it still in flight finishes first
@@ -1151,6 +1165,16 @@ int hp_batch_set_grid(hp_batch *b, int cell_blocks, int quad_blocks, int ghost_b
     return HP_OK;
 }
 
+int hp_batch_set_block(hp_batch *b, int tiles_per_block)
+{
+    if (!b || (tiles_per_block != 1 && tiles_per_block != 2 && tiles_per_block != 4))
+        return bfail(b, HP_ERR_ARG, "hp_batch_set_block: tiles_per_block must be 1, 2 or 4");
+    b->tiles_per_block = tiles_per_block;
+    batch_grids(b);
+    batch_drop_graph(b);
+    return HP_OK;
+}
+
 int hp_batch_step(hp_batch *b, int nsteps, int use_graph)
 {
     if (!b || nsteps < 0) return bfail(b, HP_ERR_ARG, "hp_batch_step: bad argument");
@@ -1175,6 +1199,7 @@ int hp_batch_step(hp_batch *b, int nsteps, int use_graph)
             if (!b->grid_quad) G.quad = round8(b->ts.nct);
             if (!b->grid_ghost) G.ghost = round8(b->ts.ngt);
             G.persist = (G.cell < b->ts.nct ? 1 : 0) | (G.quad < b->ts.nct ? 2 : 0) | (G.ghost < b->ts.ngt ? 4 : 0);
+            cellw_grid(G, b->ts.nct);
             for (int n = 0; n < nsteps; n++) enqueue_step(B, G, b->st, nullptr);
         }
     }

@@ -153,9 +153,11 @@ struct Grid {
     int quad;   // four lanes per cell, 4*TILE threads per block
     int ghost;  // ghost kernels, TILE threads per block
     int slots;  // k_dt_final, one block per slot
-    int persist;  // bits 1 cell, 2 quad, 4 ghost: the grid may hold fewer blocks than tiles
+    int persist;  // bits 1 cell, 2 quad, 4 ghost, 8 cellw: the grid may hold fewer blocks than tiles
     int single;   // one case, taken from Batch::P1/S1/M1
     int precision, myflux, myorder;
+    int tpb;      // tiles per block (1, 2 or 4) of the streaming per-cell kernels (TILE_LOOP_W)
+    int cellw;    // their blocks
 };
 
 void launch_geometry(const Mesh &m, double xmax, double ymax, double zmax, double ymin, double zmin,

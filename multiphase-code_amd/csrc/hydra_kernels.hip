@@ -63,6 +63,19 @@ __device__ __forceinline__ void tile_span(int n, int &t, int &end, int &step)
     t = (int)(((long long)n * x) >> 3) + (int)(blockIdx.x >> 3);
     end = (int)(((long long)n * (x + 1)) >> 3);
 }
+// The streaming per-cell kernels (no LDS, no barrier) take blocks of
+// blockDim.x / TILE tiles, one per wave (the batch's tiles_per_block): the
+// block's waves are its virtual blocks, consecutive in the XCD's eighth.
+__device__ __forceinline__ void tile_span_w(int n, int &t, int &end, int &step)
+{
+    // the wave's tile within the block: wave-uniform, kept scalar so the
+    // tile's case is read by scalar loads as in the one-tile blocks
+    const int tpb = (int)(blockDim.x / TILE), sub = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / TILE));
+    const int x = (int)(blockIdx.x & 7u);
+    step = (int)(gridDim.x >> 3) * tpb;
+    t = (int)(((long long)n * x) >> 3) + (int)(blockIdx.x >> 3) * tpb + sub;
+    end = (int)(((long long)n * (x + 1)) >> 3);
+}
 // MODE (a template parameter of every tile kernel): 0 tables, grid with a
 // block for every tile the tables can hold (at most one tile per block);
 // 1 tables, persistent grid (blocks loop over tiles); 2 one case carried in
@@ -70,6 +83,11 @@ __device__ __forceinline__ void tile_span(int n, int &t, int &end, int &step)
 #define TILE_LOOP(NT, N1)                                                    \
     int t_ = 0, e_ = 0, st_ = 1;                                             \
     tile_span(MODE == 2 ? ((N1) + TILE - 1) / TILE : (NT), t_, e_, st_);     \
+    for (int t = t_; t < e_; t = (MODE == 1) ? t + st_ : e_)
+// the same over blocks of several tiles (tile_span_w; lane = threadIdx.x % TILE)
+#define TILE_LOOP_W(NT, N1)                                                  \
+    int t_ = 0, e_ = 0, st_ = 1;                                             \
+    tile_span_w(MODE == 2 ? ((N1) + TILE - 1) / TILE : (NT), t_, e_, st_);   \
     for (int t = t_; t < e_; t = (MODE == 1) ? t + st_ : e_)
 
 // the case of tile t and its cell (or ghost) range; by value, so stores
@@ -2439,19 +2457,20 @@ __device__ __forceinline__ double wave_max(double v)
 // CFL bound per tile (main.F90:541-545)
 // KIND 0: primitives; 1: and the CFL bound per tile
 template <int MODE, int KIND>
-__global__ void __launch_bounds__(TILE) k_eos(Batch B)
+__global__ void __launch_bounds__(4 * TILE) k_eos(Batch B)
 {
     const bool want_dt = KIND == 1;
-    TILE_LOOP(B.cnt[0], B.n1)
+    const int l = (int)(threadIdx.x % TILE);
+    TILE_LOOP_W(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
         const State &s = C.S;
         double loc = 1.0e8;
-        if ((int)threadIdx.x < C.count)
-            eos_cell(C.P, s.cn, s.ne, C.first + (int)threadIdx.x, s.q, s.nt, want_dt ? &loc : nullptr, s.dl);
+        if (l < C.count)
+            eos_cell(C.P, s.cn, s.ne, C.first + l, s.q, s.nt, want_dt ? &loc : nullptr, s.dl);
         if (want_dt) {
             const double v = wave_min(loc);
-            if (threadIdx.x == 0) B.part[(size_t)4 * t] = v;
+            if (l == 0) B.part[(size_t)4 * t] = v;
         }
     }
 }
@@ -2535,22 +2554,24 @@ __global__ void __launch_bounds__(TILE) k_bc(Batch B, int viscous)
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(TILE) k_smooth1(Batch B)
+__global__ void __launch_bounds__(4 * TILE) k_smooth1(Batch B)
 {
-    TILE_LOOP(B.cnt[0], B.n1)
+    const int l = (int)(threadIdx.x % TILE);
+    TILE_LOOP_W(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
-        if ((int)threadIdx.x < C.count) smooth1_cell(C.S, C.M, C.first + (int)threadIdx.x);
+        if (l < C.count) smooth1_cell(C.S, C.M, C.first + l);
     }
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(TILE) k_smooth2(Batch B)
+__global__ void __launch_bounds__(4 * TILE) k_smooth2(Batch B)
 {
-    TILE_LOOP(B.cnt[0], B.n1)
+    const int l = (int)(threadIdx.x % TILE);
+    TILE_LOOP_W(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
-        if ((int)threadIdx.x < C.count) smooth2_cell(C.P, C.S, C.M, C.first + (int)threadIdx.x);
+        if (l < C.count) smooth2_cell(C.P, C.S, C.M, C.first + l);
     }
 }
 
@@ -2736,12 +2757,12 @@ void launch_grad32_geometry(const Mesh &m, const State &s, hipStream_t st)
 // in the fused step (feos), the cell's primitive recovery (eos_staged), as
 // k_flux does for the four-lanes-per-cell forms
 template <int MODE>
-__global__ void __launch_bounds__(TILE) k_cells32(Batch B, int stage, int feos)
+__global__ void __launch_bounds__(4 * TILE) k_cells32(Batch B, int stage, int feos)
 {
-    TILE_LOOP(B.cnt[0], B.n1)
+    const int l = (int)(threadIdx.x % TILE);
+    TILE_LOOP_W(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
-        const int l = (int)threadIdx.x;
         double d[4] = {0.0, 0.0, 0.0, 0.0};
         if (l < C.count) {
             const int i = C.first + l, ne = C.M.ne;
@@ -2757,7 +2778,7 @@ __global__ void __launch_bounds__(TILE) k_cells32(Batch B, int stage, int feos)
             // the tile's monitor maxima (a max does not depend on the order)
             for (int k = 0; k < 4; k++) {
                 const double v = wave_max(d[k]);
-                if (threadIdx.x == 0) B.part[(size_t)4 * t + k] = v;
+                if (l == 0) B.part[(size_t)4 * t + k] = v;
             }
         }
     }
@@ -2824,28 +2845,30 @@ __global__ void __launch_bounds__(4 * TILE) k_flux(Batch B, int stage, int feos)
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(TILE) k_rk_update(Batch B, int stage)
+__global__ void __launch_bounds__(4 * TILE) k_rk_update(Batch B, int stage)
 {
-    TILE_LOOP(B.cnt[0], B.n1)
+    const int l = (int)(threadIdx.x % TILE);
+    TILE_LOOP_W(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
-        if ((int)threadIdx.x < C.count) rk_cell(C.S, C.M, C.first + (int)threadIdx.x, stage);
+        if (l < C.count) rk_cell(C.S, C.M, C.first + l, stage);
     }
 }
 
 // cu = cn; want_diffs: the tile's max|cn-cu| of variables 1..4 into part
 template <int MODE>
-__global__ void __launch_bounds__(TILE) k_end_step(Batch B, int want_diffs)
+__global__ void __launch_bounds__(4 * TILE) k_end_step(Batch B, int want_diffs)
 {
-    TILE_LOOP(B.cnt[0], B.n1)
+    const int l = (int)(threadIdx.x % TILE);
+    TILE_LOOP_W(B.cnt[0], B.n1)
     {
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
         double d[4] = {0.0, 0.0, 0.0, 0.0};
-        if ((int)threadIdx.x < C.count) end_cell(C.S, C.first + (int)threadIdx.x, d);
+        if (l < C.count) end_cell(C.S, C.first + l, d);
         if (want_diffs) {
             for (int k = 0; k < 4; k++) {
                 const double v = wave_max(d[k]);
-                if (threadIdx.x == 0) B.part[(size_t)4 * t + k] = v;
+                if (l == 0) B.part[(size_t)4 * t + k] = v;
             }
         }
     }
@@ -2888,9 +2911,9 @@ __global__ void __launch_bounds__(TILE) k_health(Batch B, int *flags)
 void launch_eos(const Batch &B, const Grid &G, int want_dt, hipStream_t st)
 {
     if (want_dt)
-        hipLaunchKernelGGL(PICK(1, k_eos, 1), dim3(G.cell), dim3(TILE), 0, st, B);
+        hipLaunchKernelGGL(PICK(8, k_eos, 1), dim3(G.cellw), dim3(TILE * G.tpb), 0, st, B);
     else
-        hipLaunchKernelGGL(PICK(1, k_eos, 0), dim3(G.cell), dim3(TILE), 0, st, B);
+        hipLaunchKernelGGL(PICK(8, k_eos, 0), dim3(G.cellw), dim3(TILE * G.tpb), 0, st, B);
 }
 
 
@@ -2911,8 +2934,8 @@ void launch_bc(const Batch &B, const Grid &G, int viscous, hipStream_t st)
 
 void launch_smooth(const Batch &B, const Grid &G, hipStream_t st)
 {
-    hipLaunchKernelGGL(PICK(1, k_smooth1), dim3(G.cell), dim3(TILE), 0, st, B);
-    hipLaunchKernelGGL(PICK(1, k_smooth2), dim3(G.cell), dim3(TILE), 0, st, B);
+    hipLaunchKernelGGL(PICK(8, k_smooth1), dim3(G.cellw), dim3(TILE * G.tpb), 0, st, B);
+    hipLaunchKernelGGL(PICK(8, k_smooth2), dim3(G.cellw), dim3(TILE * G.tpb), 0, st, B);
 }
 
 void launch_viscous(const Batch &B, const Grid &G, hipStream_t st)
@@ -2948,7 +2971,7 @@ void launch_flux(const Batch &B, const Grid &G, int stage, int feos, hipStream_t
             // each face's flux once, then one thread per cell: its sums,
             // source, update and primitive recovery
             hipLaunchKernelGGL(PICK(2, k_face32), g, b, 0, st, B);
-            hipLaunchKernelGGL(PICK(1, k_cells32), dim3(G.cell), dim3(TILE), 0, st, B, stage, feos);
+            hipLaunchKernelGGL(PICK(8, k_cells32), dim3(G.cellw), dim3(TILE * G.tpb), 0, st, B, stage, feos);
         }
     } else {
         if (G.myflux == 1) hipLaunchKernelGGL(PICK(2, k_flux, 1, false), g, b, 0, st, B, stage, feos);
@@ -2958,12 +2981,12 @@ void launch_flux(const Batch &B, const Grid &G, int stage, int feos, hipStream_t
 
 void launch_rk_update(const Batch &B, const Grid &G, int stage, hipStream_t st)
 {
-    hipLaunchKernelGGL(PICK(1, k_rk_update), dim3(G.cell), dim3(TILE), 0, st, B, stage);
+    hipLaunchKernelGGL(PICK(8, k_rk_update), dim3(G.cellw), dim3(TILE * G.tpb), 0, st, B, stage);
 }
 
 void launch_end_step(const Batch &B, const Grid &G, int want_diffs, hipStream_t st)
 {
-    hipLaunchKernelGGL(PICK(1, k_end_step), dim3(G.cell), dim3(TILE), 0, st, B, want_diffs);
+    hipLaunchKernelGGL(PICK(8, k_end_step), dim3(G.cellw), dim3(TILE * G.tpb), 0, st, B, want_diffs);
 }
 
 // self-check of the shared-divisor division: pairs where divide() and the

@@ -48,6 +48,7 @@ EXPORTS = ["hp_create", "hp_destroy", "hp_upload_state", "hp_download_state", "h
            "hp_end_step", "hp_step", "hp_synchronize", "hp_last_error", "hp_get_array",
            "hp_set_array", "hp_last_timing", "hp_probe_cell", "hp_health", "hp_batch_health", "hp_check_division",
            "hp_batch_create", "hp_batch_destroy", "hp_batch_add", "hp_batch_remove", "hp_batch_set_grid",
+           "hp_batch_set_block",
            "hp_batch_step", "hp_batch_synchronize", "hp_batch_last_ms", "hp_batch_last_error",
            "hp_case_bytes", "hp_set_donors", "hp_apply_donors", "hp_dt_min_noalloc",
            "hp_stream_bandwidth", "hp_step_diffs", "hp_last_diffs", "hp_set_flux_timing"]
@@ -92,6 +93,7 @@ def lib():
     L.hp_batch_add.argtypes = [vp, vp, ip]
     L.hp_batch_remove.argtypes = [vp, vp]
     L.hp_batch_set_grid.argtypes = [vp, c_int, c_int, c_int]
+    L.hp_batch_set_block.argtypes = [vp, c_int]
     L.hp_batch_step.argtypes = [vp, c_int, c_int]
     L.hp_batch_synchronize.argtypes = [vp]
     L.hp_batch_last_ms.argtypes = [vp, dp]

@@ -51,6 +51,7 @@ class Batch:
         self.max_cases = int(max_cases)
         self.members: dict[int, Solver] = {}
         self.grid = (0, 0, 0)
+        self.block = 1
 
     def _chk(self, rc):
         if rc != 0:
@@ -71,6 +72,12 @@ class Batch:
     def set_grid(self, cell: int, quad: int, ghost: int):
         self._chk(self.L.hp_batch_set_grid(self.h, int(cell), int(quad), int(ghost)))
         self.grid = (cell, quad, ghost)
+
+    def set_block(self, tiles_per_block: int):
+        """Tiles of 64 cells per block (1, 2 or 4) for the streaming per-cell
+        kernels (hp_batch_set_block)."""
+        self._chk(self.L.hp_batch_set_block(self.h, int(tiles_per_block)))
+        self.block = int(tiles_per_block)
 
     def step(self, n: int = 1, graph: bool = True):
         self._chk(self.L.hp_batch_step(self.h, int(n), 1 if graph else 0))
@@ -161,6 +168,7 @@ def capacity(meshes) -> tuple[int, int, int]:
 # --------------------------------------------------------------------------
 
 GRID_CHOICES = [0, 2 * CUS, 4 * CUS, 8 * CUS, 16 * CUS]
+BLOCK_CHOICES = [1, 2, 4]
 
 
 def _time_steps(b: Batch, steps: int) -> float:
@@ -176,10 +184,11 @@ def tune(specs, precision: int = 1, steps: int = 3, myflux: int = 2, log=None) -
     Builds a scratch copy of the ensemble (so the real cases do not move),
     then walks the choices one family at a time (coordinate descent): cell
     order, then cell-kernel blocks, face-kernel blocks and ghost-kernel
-    blocks, each from GRID_CHOICES (0 = one block per tile).  The cell
-    orders tried: as generated, Z order (morton) and y-z slabs (slabs).
-    Returns {"morton", "slabs", "grid": (cell, quad, ghost), "ms_per_step",
-    "trials"}."""
+    blocks, each from GRID_CHOICES (0 = one block per tile), then the block
+    size of the streaming per-cell kernels (BLOCK_CHOICES tiles of 64 cells
+    per block).  The cell orders tried: as generated, Z order (morton) and
+    y-z slabs (slabs).  Returns {"morton", "slabs", "grid": (cell, quad,
+    ghost), "block", "ms_per_step", "trials"}."""
     trials = []
 
     def build(order):
@@ -224,10 +233,20 @@ def tune(specs, precision: int = 1, steps: int = 3, myflux: int = 2, log=None) -
                 log(f"tune: grid={tuple(trial)} {t:.3f} ms/step")
             if t < best_t:
                 best_t, grid = t, trial
+    b.set_grid(*grid)
+    block = 1
+    for tpb in BLOCK_CHOICES[1:]:
+        b.set_block(tpb)
+        t = _time_steps(b, steps)
+        trials.append((dict(order, grid=tuple(grid), block=tpb), t))
+        if log:
+            log(f"tune: grid={tuple(grid)} block={tpb} {t:.3f} ms/step")
+        if t < best_t:
+            best_t, block = t, tpb
     b.close()
     for s in sols:
         s.close()
-    return dict(order, grid=tuple(grid), ms_per_step=best_t, trials=trials)
+    return dict(order, grid=tuple(grid), block=block, ms_per_step=best_t, trials=trials)
 
 
 def run_ensemble(specs, precision: int = 1, tuning: dict | None = None, graph: bool = True,
@@ -240,6 +259,7 @@ def run_ensemble(specs, precision: int = 1, tuning: dict | None = None, graph: b
     cases = [build_case(s, precision, tuning["morton"], myflux, tuning.get("slabs", False)) for s in specs]
     b = Batch(*capacity([c[0] for c in cases]))
     b.set_grid(*tuning["grid"])
+    b.set_block(tuning.get("block", 1))
     sols, slots = [], []
     for (m, p, cu, gp, _) in cases:
         s = Solver(m, p)

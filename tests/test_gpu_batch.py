@@ -32,9 +32,10 @@ def alone(case, nsteps):
     return out
 
 
-def in_batch(cases, nsteps, graph=True, grid=(0, 0, 0)):
+def in_batch(cases, nsteps, graph=True, grid=(0, 0, 0), block=1):
     b = HB.Batch(*HB.capacity([c[0] for c in cases]))
     b.set_grid(*grid)
+    b.set_block(block)
     sols = []
     for (m, p, cu, gp, _) in cases:
         s = Solver(m, p)
@@ -104,6 +105,24 @@ def test_launch_shapes_give_same_results(precision):
         got = in_batch(cases, 3, grid=grid)
         for k in range(len(cases)):
             assert same(got[k], ref[k]), (grid, k)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_block_sizes_give_same_results(precision):
+    """Blocks of 2 or 4 tiles for the streaming per-cell kernels (one tile
+    per wave), with one block per tile of the other families or persistent
+    grids, eager and graph: the same bits as blocks of one tile."""
+    cases = [HB.build_case(s, precision) for s in specs()]
+    ref = in_batch(cases, 3)
+    for block in (2, 4):
+        for grid, graph in [((0, 0, 0), True), ((0, 0, 0), False), ((8, 8, 8), True), ((24, 16, 8), False)]:
+            got = in_batch(cases, 3, graph=graph, grid=grid, block=block)
+            for k in range(len(cases)):
+                assert same(got[k], ref[k]), (block, grid, graph, k)
+    b = HB.Batch(1, 1000, 1000)
+    with pytest.raises(Exception):
+        b.set_block(3)
+    b.close()
 
 
 def test_join_and_leave_with_one_graph():
