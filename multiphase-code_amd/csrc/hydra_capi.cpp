@@ -392,7 +392,7 @@ int hp_create(hp_ctx **out, int nodes, int neles, int nghosts, const double *x, 
         if (!hflist.empty()) {
             ALLOC(dfslot, hfslot.size()) ALLOC(dfbase, hfbase.size()) ALLOC(dflist, hflist.size())
             ALLOC(s.fflux, (size_t)FACE32_N * hflist.size())
-            ALLOC(s.fgeom, (size_t)12 * hflist.size())
+            ALLOC(s.fgeom, (size_t)FGEO32_N * hflist.size())
             ALLOC(s.src32, (size_t)SRC32_N * ne)
         }
     } else {
@@ -1282,7 +1282,7 @@ size_t hp_case_bytes(int nodes, int neles, int nghosts, int precision)
     b += (8 + 10) * ne * (precision == 1 ? sizeof(float) : D);  // viscous fluxes, JST dissipation
     if (precision == 1) {
         b += (size_t)R32_N * nt * sizeof(float) + (size_t)CGEO32_N * ne * sizeof(float);  // FP32-bulk records, geometry
-        b += (9 * ne + 1) * I + 4 * ne * (FACE32_N + 12) * sizeof(float);  // face tables (HLLC-Batten; at most)
+        b += (9 * ne + 1) * I + 4 * ne * (FACE32_N + FGEO32_N) * sizeof(float);  // face tables (HLLC-Batten; at most)
         b += SRC32_N * ne * sizeof(float);                                   // its source terms
     }
     else b += SRC_N * ne * D;                                     // interphase source terms

@@ -32,6 +32,8 @@ enum { SC_DT, SC_TIME, SC_DTMIN, SC_N };
 enum { R32_PRIM = 0, R32_GRAD = 13, R32_PHI = 46, R32_N = 48 };
 // floats per face entry of State::fflux: the 10 flux components, 2 pad
 enum { FACE32_N = 12 };
+// floats per listed face of State::fgeom
+enum { FGEO32_N = 9 };
 // the FP32-bulk HLLC-Batten form's interphase source, per cell, as the five
 // FP32 terms the stage update widens and multiplies by the volume (variable
 // 1, variables 2 and 7, 3 and 8, 4 and 9, variable 6)
@@ -91,8 +93,8 @@ struct State {
     // components, 2 pad); fslot (4 x ne, face-major) gives each cell
     // face its entry as (k << 1) | 1 when the cell takes the listed values
     // with the opposite sign; fgeom holds each listed face's FP32 geometry,
-    // 12 floats (FaceGeo32 order: area vector, face-centroid offsets from
-    // both cells, the centroid offset between them), computed at set-up
+    // FGEO32_N floats (the area vector, the face centroid's offsets from both
+    // cells' centroids), computed at set-up
     const int *fslot, *fbase, *flist;
     float *fflux, *fgeom;
     float *src32;           // FP32-bulk HLLC-Batten: SRC32_N x ne source terms, written by the gradient

@@ -32,6 +32,11 @@
 
 namespace hydra {
 
+// four floats as the compiler's native vector (non-temporal stores of
+// 16-byte pieces: the stage's records, entries and state are written once
+// and read by the next launch, so they bypass the caches the gathers use)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 
@@ -472,10 +477,11 @@ __device__ __forceinline__ void eos_cell(const Params &P, double *cn, int ne, in
                                          double *dtloc, const double *dl, double *cu = nullptr)
 {
     double c[10], v[Q_N], nv[10];
-    for (int k = 0; k < 10; k++) c[k] = cn[(size_t)k * ne + i];
+    // streamed once: non-temporal loads and stores
+    for (int k = 0; k < 10; k++) c[k] = __builtin_nontemporal_load(&cn[(size_t)k * ne + i]);
     eos_prims(P, c, v, nv);
-    for (int k = 0; k < Q_N; k++) QV(k, i) = v[k];
-    for (int k = 0; k < 10; k++) cn[(size_t)k * ne + i] = nv[k];
+    for (int k = 0; k < Q_N; k++) __builtin_nontemporal_store(v[k], &QV(k, i));
+    for (int k = 0; k < 10; k++) __builtin_nontemporal_store(nv[k], &cn[(size_t)k * ne + i]);
     if (cu)
         for (int k = 0; k < 10; k++) cu[(size_t)k * ne + i] = nv[k];
     if (dtloc) {
@@ -1275,18 +1281,20 @@ __device__ __forceinline__ double rk_finish(const State &s, const TD *d24, size_
         s.rhs[o] = r;
         return 0.0;
     }
+    // the stage's state is streamed through once: non-temporal loads and
+    // stores keep it out of the caches the gathers use
     const double rr = r - (double)d24[o];
-    const double cu = s.cu[o];
-    double cn = s.cn[o];
+    const double cu = __builtin_nontemporal_load(&s.cu[o]);
+    double cn = __builtin_nontemporal_load(&s.cn[o]);
     if (stage == 1) {
         cn = cu - (11.0 / 20.0) * dtbyvol * rr;
-        s.cn1[o] = cn;
+        __builtin_nontemporal_store(cn, &s.cn1[o]);
     } else if (stage == 2) {
         cn = (3.0 / 8.0) * cu + (5.0 / 8.0) * (cn - (11.0 / 20.0) * dtbyvol * rr);
     } else if (stage == 3) {
         cn = (4.0 / 9.0) * cu + (5.0 / 9.0) * (cn - (11.0 / 20.0) * dtbyvol * rr);
     } else {
-        cn = (111.0 / 1331.0) * cu + (260.0 / 1331.0) * s.cn1[o] +
+        cn = (111.0 / 1331.0) * cu + (260.0 / 1331.0) * __builtin_nontemporal_load(&s.cn1[o]) +
              (960.0 / 1331.0) * (cn - (11.0 / 20.0) * dtbyvol * rr);
     }
     if (store) s.cn[o] = cn;
@@ -1342,17 +1350,18 @@ __device__ __forceinline__ void eos_staged(const Params &P, const State &s, int 
     eos_prims(P, c, v, nv);
     const int ne = s.ne;
     const size_t nt = s.nt;
-    for (int k = 0; k < Q_N; k++) s.q2[(size_t)k * nt + i] = v[k];
+    // streamed stores (non-temporal), as the stage update's
+    for (int k = 0; k < Q_N; k++) __builtin_nontemporal_store(v[k], &s.q2[(size_t)k * nt + i]);
     const unsigned char fl = s.dflag ? s.dflag[i] : 0;
     if (fl & 1)
         for (int k = 0; k < 10; k++) s.cnd[(size_t)k * ne + i] = c[k];
-    for (int k = 0; k < 10; k++) s.cn[(size_t)k * ne + i] = nv[k];
+    for (int k = 0; k < 10; k++) __builtin_nontemporal_store(nv[k], &s.cn[(size_t)k * ne + i]);
     if (end && !(fl & 2)) {
         // the monitor (main.F90:603-612) against the step's starting cu, and
         // cu = cn; an overset receptor's final value, its monitor difference
         // and its cu come from the gather that follows, which reads its cu
         for (int k = 0; k < 4; k++) d[k] = fabs(nv[k] - s.cu[(size_t)k * ne + i]);
-        for (int k = 0; k < 10; k++) s.cu[(size_t)k * ne + i] = nv[k];
+        for (int k = 0; k < 10; k++) __builtin_nontemporal_store(nv[k], &s.cu[(size_t)k * ne + i]);
     }
 }
 
@@ -1644,7 +1653,7 @@ __device__ __forceinline__ void grad_cell_f32(const Params &P, const State &s, c
         for (int k = 0; k < 4; k++) dph[k] = (float)(QV(Q_PHIG, nb[k]) - phi_i);
         cell_source32(P, s, m, i, dx, dy, dz, dph, T);
 #pragma unroll
-        for (int j = 0; j < SRC32_N; j++) s.src32[(size_t)j * ne + i] = T[j];
+        for (int j = 0; j < SRC32_N; j++) __builtin_nontemporal_store(T[j], &s.src32[(size_t)j * ne + i]);
     }
     const float idn = frcp(det3f(a1, b1, b2, b1, a2, b3, b2, b3, a3));
     const int slot[13] = {Q_UG, Q_VG, Q_WG, Q_TG, Q_UP, Q_VP, Q_WP, Q_TP, Q_PG, Q_PP, Q_PHIG, Q_ROG, Q_ROP};
@@ -2618,7 +2627,9 @@ __global__ void __launch_bounds__(TILE) k_grad32(Batch B)
                 grad_cell_f32(C.P, C.S, C.M, C.first + lane, reinterpret_cast<float *>(stage32) + R32_N * lane);
             __syncthreads();
             float4 *out = reinterpret_cast<float4 *>(C.S.rec32) + (size_t)(R32_N / 4) * C.first;
-            for (int k = lane; k < C.count * (R32_N / 4); k += TILE) out[k] = stage32[k];
+            for (int k = lane; k < C.count * (R32_N / 4); k += TILE)
+                __builtin_nontemporal_store(*reinterpret_cast<const f32x4 *>(&stage32[k]),
+                                            reinterpret_cast<f32x4 *>(&out[k]));
             if (MODE == 1) __syncthreads();  // the next tile reuses the staging
         }
     }
@@ -2670,9 +2681,9 @@ __global__ void __launch_bounds__(4 * TILE) k_face32(Batch B)
             const int lf = l < nf ? l : nf - 1;
             const int fc = C.S.flist[base + lf], i = fc >> 2;
             const int neigh = C.M.nc[(size_t)(fc & 3) * C.M.ne + i];
-            const float4 *gs = reinterpret_cast<const float4 *>(C.S.fgeom) + (size_t)3 * (base + lf);
-            const float4 g0 = gs[0], g1 = gs[1], g2 = gs[2];
-            const FaceGeo32 g = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w, g2.x, g2.y, g2.z, g2.w};
+            const float3 *gs = reinterpret_cast<const float3 *>(C.S.fgeom) + (size_t)3 * (base + lf);
+            const float3 g0 = gs[0], g1 = gs[1], g2 = gs[2];
+            const FaceGeo32 g = {g0.x, g0.y, g0.z, g1.x, g1.y, g1.z, g2.x, g2.y, g2.z, 0.0f, 0.0f, 0.0f};
             float F[10];
             if (C.P.myorder == 2) {
                 const float4 *rec = reinterpret_cast<const float4 *>(C.S.rec32);
@@ -2699,10 +2710,10 @@ __global__ void __launch_bounds__(4 * TILE) k_face32(Batch B)
                 face_flux_f32<2>(C.P, C.S, C.M, i, neigh, g, F);
             }
             if (l < nf) {
-                float4 *dst = reinterpret_cast<float4 *>(C.S.fflux) + (size_t)FACE32_N / 4 * (base + l);
-                dst[0] = make_float4(F[0], F[1], F[2], F[3]);
-                dst[1] = make_float4(F[4], F[5], F[6], F[7]);
-                dst[2] = make_float4(F[8], F[9], 0.0f, 0.0f);
+                f32x4 *dst = reinterpret_cast<f32x4 *>(C.S.fflux) + (size_t)FACE32_N / 4 * (base + l);
+                __builtin_nontemporal_store((f32x4){F[0], F[1], F[2], F[3]}, &dst[0]);
+                __builtin_nontemporal_store((f32x4){F[4], F[5], F[6], F[7]}, &dst[1]);
+                __builtin_nontemporal_store((f32x4){F[8], F[9], 0.0f, 0.0f}, &dst[2]);
             }
         }
     }
@@ -2715,10 +2726,10 @@ __global__ void k_face32_geom(Mesh m, State s, int nf)
     if (k >= nf) return;
     const int fc = s.flist[k], i = fc >> 2, f = fc & 3;
     const FaceGeo32 g = face_geo32(m, i, f, m.nc[(size_t)f * m.ne + i]);
-    float4 *d = reinterpret_cast<float4 *>(s.fgeom) + (size_t)3 * k;
-    d[0] = make_float4(g.scx, g.scy, g.scz, g.xf0);
-    d[1] = make_float4(g.yf0, g.zf0, g.xfn, g.yfn);
-    d[2] = make_float4(g.zfn, g.dx, g.dy, g.dz);
+    float3 *d = reinterpret_cast<float3 *>(s.fgeom) + (size_t)3 * k;
+    d[0] = make_float3(g.scx, g.scy, g.scz);
+    d[1] = make_float3(g.xf0, g.yf0, g.zf0);
+    d[2] = make_float3(g.xfn, g.yfn, g.zfn);
 }
 
 void launch_face32_geometry(const Mesh &m, const State &s, int nf, hipStream_t st)
