@@ -218,7 +218,7 @@ def summarize_counters(read_csv, write_csv, calib_mb):
         print(f"| `{name}` | {nrd[k]} | {mix} | {r:.1f} | {w:.1f} |")
         if name.startswith("hydra::k_flux"):
             flux = int(round((r + w) * 1e6))
-        if name.startswith("hydra::k_face32") or name.startswith("hydra::k_cells32"):
+        if name.startswith("hydra::k_face32<") or name.startswith("hydra::k_cells32<"):
             face32 = (face32 or 0) + int(round((r + w) * 1e6))
     if flux is not None:
         print(f"\ntraffic_bytes_per_launch k_flux: {flux}")
