@@ -582,8 +582,14 @@ int hp_dt_min(hp_ctx *c, double *dtmin)
     std::vector<double> part(4 * (size_t)c->ts.nct);
     HIPCHK(c, hipMemcpyAsync(part.data(), c->ts.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    // the tiles folded in cell order as the reference's MIN loop (a tile
+    // holding a NaN bound restarts the fold: its part is independent of
+    // what came before)
     double r = 1.0e8;
-    for (int t = 0; t < c->ts.nct; t++) r = part[(size_t)4 * t] < r ? part[(size_t)4 * t] : r;
+    for (int t = 0; t < c->ts.nct; t++) {
+        const double m = part[(size_t)4 * t];
+        r = part[(size_t)4 * t + 1] != 0.0 ? m : (r < m ? r : m);
+    }
     *dtmin = r;
     return HP_OK;
 }

@@ -502,6 +502,35 @@ __device__ __forceinline__ double wave_min(double v)
     return v;
 }
 
+// A tile's share of the CFL minimum, which the reference takes over the
+// cells in order (dtmin = MIN(dtmin, bound), main.F90:545; the serial
+// oracle folds dtmin < bound ? dtmin : bound).  Without a NaN bound the
+// fold takes the tile's minimum m as dmin(dtmin, m): part[4t] = m,
+// part[4t+1] = 0.  A NaN bound restarts the fold, so a tile holding one
+// yields the minimum of the bounds after its last NaN (NaN if that is the
+// tile's last cell) whatever came before: part[4t+1] = 1.  Lane l holds
+// the bound of cell l of the tile's count.
+__device__ __forceinline__ void tile_dt_part(const Batch &B, int t, double loc, int l, int count)
+{
+    const bool valid = l < count;
+    const unsigned long long nanm = __ballot(valid && !(loc == loc));
+    if (nanm == 0ull) {
+        const double v = wave_min(loc);
+        if (l == 0) {
+            B.part[(size_t)4 * t] = v;
+            B.part[(size_t)4 * t + 1] = 0.0;
+        }
+    } else {
+        const int j = 63 - __clzll((long long)nanm);  // the tile's last NaN
+        double v = wave_min((valid && l > j) ? loc : __builtin_inf());
+        if (j == count - 1) v = __builtin_nan("");
+        if (l == 0) {
+            B.part[(size_t)4 * t] = v;
+            B.part[(size_t)4 * t + 1] = 1.0;
+        }
+    }
+}
+
 
 
 
@@ -2477,21 +2506,52 @@ __global__ void __launch_bounds__(4 * TILE) k_eos(Batch B)
         double loc = 1.0e8;
         if (l < C.count)
             eos_cell(C.P, s.cn, s.ne, C.first + l, s.q, s.nt, want_dt ? &loc : nullptr, s.dl);
-        if (want_dt) {
-            const double v = wave_min(loc);
-            if (l == 0) B.part[(size_t)4 * t] = v;
-        }
+        if (want_dt) tile_dt_part(B, t, loc, l, C.count);
     }
 }
 
 // per case slot (one block each): dtmin over its tiles, dt = cfl*dtmin,
-// time += dt, total += 1 (main.F90:520-523, :555-556); a min does not
-// depend on the order it is taken in
+// time += dt, total += 1 (main.F90:520-523, :555-556).  Without a NaN bound
+// the minimum does not depend on the order it is taken in; with one the
+// tiles are folded in cell order (tile_dt_part): each thread folds a
+// contiguous run of tiles, then the first thread folds the runs in order.
 __global__ void __launch_bounds__(1024) k_dt_final(Batch B)
 {
     const int c = blockIdx.x;
     const int t0 = B.slot_t0[c], t1 = B.slot_t1[c];
     if (t0 >= t1) return;
+    int nan_tiles = 0;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += (int)blockDim.x) nan_tiles |= B.part[(size_t)4 * t + 1] != 0.0;
+    if (__syncthreads_or(nan_tiles)) {
+        __shared__ double run_m[1024];
+        __shared__ int run_h[1024];
+        const int n = t1 - t0, per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
+        const int a = t0 + (int)threadIdx.x * per, e = min(a + per, t1);
+        double m = 0.0;
+        int h = -1;  // -1: empty run
+        for (int t = a; t < e; t++) {
+            const double tm = B.part[(size_t)4 * t];
+            if (B.part[(size_t)4 * t + 1] != 0.0) { m = tm; h = 1; }
+            else if (h < 0) { m = tm; h = 0; }
+            else m = dmin(m, tm);
+        }
+        run_m[threadIdx.x] = m;
+        run_h[threadIdx.x] = h;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = 1.0e8;
+            for (int k = 0; k < (int)blockDim.x; k++)
+                if (run_h[k] == 1) r = run_m[k];
+                else if (run_h[k] == 0) r = r < run_m[k] ? r : run_m[k];
+            const State &s = B.S[c];
+            const double dt = r * B.P[c].cfl;
+            s.scal[SC_DTMIN] = r;
+            s.scal[SC_DT] = dt;
+            s.scal[SC_TIME] = s.scal[SC_TIME] + dt;
+            s.itot[0] = s.itot[0] + 1;
+        }
+        return;
+    }
     double v = 1.0e8;
     // eight independent loads in flight per thread (the partials of a
     // 5M-cell case are 78k tiles; one load at a time left this launch
@@ -2529,8 +2589,7 @@ __global__ void __launch_bounds__(TILE) k_dt_from_prims(Batch B)
         const TileCase C = tile_case<MODE>(B, B.ct, t, B.n1);
         double loc = 1.0e8;
         if ((int)threadIdx.x < C.count) loc = dt_cell(C.P, C.S, C.first + (int)threadIdx.x);
-        const double v = wave_min(loc);
-        if (threadIdx.x == 0) B.part[(size_t)4 * t] = v;
+        tile_dt_part(B, t, loc, (int)threadIdx.x, C.count);
     }
 }
 

@@ -22,6 +22,46 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
+def test_breakdown_steps_bitwise():
+    """A case whose state breaks down (the 10 x 9 x 8 box with the wall
+    cylinder: 34 cells hold NaN primitives after two steps, as the
+    reference's own step gives them).  The CFL minimum over NaN bounds
+    depends on the order it is taken in; the GPU folds its tiles in cell
+    order as the reference's MIN loop does, so eight steps still equal the
+    oracle's (NaN where the oracle has NaN), and so does hp_dt_min on the
+    broken state."""
+    m = M.box_mesh(10, 9, 8, jitter=0.1, seed=3, hole=(0.05, 0.0, 0.16))
+    p = M.water_air_params(cfl=0.5, viscous_gas=True)
+    cu0, gp0 = M.sphere_state(m, p, [(0.0, 0.1, 0.0, 0.22)])
+    s = solver_cls()(m, p)
+    o = O.Oracle(m, p)
+    try:
+        s.upload(cu0, gp0, 0.0, 0)
+        o.set_state(cu0, gp0, 0.0, 0)
+        broke = False
+        for k in range(8):
+            s.step(1)
+            o.run(1)
+            cu, gp, t, tot = s.download()
+            cu_o, gp_o, t_o, tot_o = o.get_state()
+            assert tot == tot_o and (t == t_o or (np.isnan(t) and np.isnan(t_o))), (k, t, t_o)
+            assert_same(f"cu step {k + 1}", cu, cu_o)
+            assert_same(f"ghost prims step {k + 1}", gp, gp_o)
+            broke = broke or bool(np.isnan(cu).any())
+        assert broke, "the case was meant to break down"
+        s.upload(cu0, gp0, 0.0, 0)
+        o.set_state(cu0, gp0, 0.0, 0)
+        s.step(2)
+        o.run(2)
+        s.primitive()
+        o.call("primitive")
+        assert np.isnan(o.array("q")[:14, :m.neles]).any()
+        d_g, d_o = s.dt_min(), o.call("dtmin")
+        assert d_g == d_o or (np.isnan(d_g) and np.isnan(d_o)), (d_g, d_o)
+    finally:
+        s.close()
+
+
 def solver_cls():
     from hydraphase import Solver
     return Solver
