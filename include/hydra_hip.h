@@ -246,6 +246,9 @@ int hp_stream_bandwidth(size_t bytes, int reps, double *gbs, double *ms_per_copy
  *   dl(neles): device minimum edge lengths (hp_get_array id 12)
  *   dtmin: one device double receiving min over cells of
  *          dl / (max(ag, ap) + max(|u_g|, |u_p|))
+ *          (hp_dt_min's value whenever no bound is NaN; on a state that
+ *          has broken down hp_dt_min follows the reference's cell order
+ *          through the NaN bounds and this minimum does not)
  * The call allocates nothing: its scratch (one partial minimum per 256
  * cells) is a fixed 2 MiB device array of the library that each GPU holds
  * from set-up (hp_create, or the first call on the device) until the

@@ -80,8 +80,10 @@ k_dt_part_noalloc(const double *q, size_t ldq, const double *dl, int n, double g
     }
 }
 
-// The minimum of the nparts partials into out[0]; a minimum does not depend
-// on the order it is taken in, so this equals hp_dt_min's host reduction
+// The minimum of the nparts partials into out[0]; without a NaN bound a
+// minimum does not depend on the order it is taken in, so this equals
+// hp_dt_min's host reduction (with NaN bounds, a state that has broken
+// down, hp_dt_min follows the reference's cell order and this does not)
 __global__ void __launch_bounds__(1024) k_dt_final_noalloc(const double *part, int nparts, double *out)
 {
     double v = 1.0e8;
