@@ -407,23 +407,23 @@ def run_ensemble_steps(mode="graph", steps=10, warmup=2):
 SERVE_MAX_WAIT_S = 4.0
 
 
-def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05, max_wait_s=SERVE_MAX_WAIT_S):
+def serve_bench(log, n_requests=48, rate=20.0, sample_frac=0.05, max_wait_s=SERVE_MAX_WAIT_S, seed=7):
     """Poisson arrivals of cases (n 10..70, 50..500 steps), admitted smallest
     work first within a 200 GB HBM budget while at most 1M active cells are
     resident (a request waiting longer than max_wait_s goes first), and
-    stepped together (FP32 bulk)."""
+    stepped together (FP32 bulk).  sample_frac = 0: no error sample."""
     from hydraphase import serve as SV
-    reqs = SV.poisson_requests(n_requests, rate=rate, seed=7, nmin=10, nmax=70, steps=(50, 500), precision=1)
+    reqs = SV.poisson_requests(n_requests, rate=rate, seed=seed, nmin=10, nmax=70, steps=(50, 500), precision=1)
     srv = SV.Server(budget_bytes=200e9, max_cases=256, chunk=25, policy="shortest", fill_cells=1_000_000,
                     max_wait_s=max_wait_s)
     try:
         st = srv.run(reqs)
     finally:
         srv.close()
-    k = max(1, int(round(sample_frac * n_requests)))
+    k = max(1, int(round(sample_frac * n_requests))) if sample_frac > 0 else 0
     done = [r for r in reqs if r.error is None]  # requests retired as failed carry no result
     errs, floors = [], []
-    for r in done[:: max(1, len(done) // k)][:k]:
+    for r in (done[:: max(1, len(done) // k)][:k] if k else []):
         ref = SV.alone(r, precision=0)
         errs.append(float(SV.mixture_rel_l2(r.result[0], ref[0]).max()))
         floors.append(float(SV.mixture_rel_l2(SV.alone(r, precision=0, noise=2.0 ** -24)[0], ref[0]).max()))
@@ -559,6 +559,12 @@ def main():
     if not args.no_serve:
         out["serve"] = serve_bench(log)
         log(f"# serve: p50 {out['serve']['tts_p50_s']:.2f} s p99 {out['serve']['tts_p99_s']:.2f} s")
+        # the same request mix arriving below the GPU's capacity (about 2.8
+        # requests/s for this mix): time to solution without the queueing
+        # that 20 requests/s imposes
+        out["serve_light"] = serve_bench(log, n_requests=16, rate=1.5, sample_frac=0.0, seed=8)
+        log(f"# serve at 1.5/s: p50 {out['serve_light']['tts_p50_s']:.2f} s "
+            f"p99 {out['serve_light']['tts_p99_s']:.2f} s")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(m, p, cu, gp, donors, args.cpu_seconds)
     print(json.dumps(out), flush=True)
