@@ -1026,9 +1026,9 @@ __device__ __forceinline__ void grad_cell(const State &s, const Mesh &m, int i)
         const double gy = divide(det3(a1, b1, b2, c1, c2, c3, b2, b3, a3), D);
         const double gz = divide(det3(a1, b1, b2, b1, a2, b3, c1, c2, c3), D);
         if (sI < 11) {
-            s.grad[(size_t)(3 * sI + 0) * ne + i] = gx;
-            s.grad[(size_t)(3 * sI + 1) * ne + i] = gy;
-            s.grad[(size_t)(3 * sI + 2) * ne + i] = gz;
+            __builtin_nontemporal_store(gx, &s.grad[(size_t)(3 * sI + 0) * ne + i]);
+            __builtin_nontemporal_store(gy, &s.grad[(size_t)(3 * sI + 1) * ne + i]);
+            __builtin_nontemporal_store(gz, &s.grad[(size_t)(3 * sI + 2) * ne + i]);
         }
         const int li = (sl == Q_PG) ? 0 : (sl == Q_PP) ? 1 : (sl == Q_ROG) ? 2 : (sl == Q_ROP) ? 3 : -1;
         if (li >= 0) {
@@ -1066,7 +1066,7 @@ __device__ __forceinline__ void grad_cell(const State &s, const Mesh &m, int i)
             ph = (v == 0 && f == 0) ? pv : dmin(ph, pv);
         }
     }
-    s.phi[i] = ph;
+    __builtin_nontemporal_store(ph, &s.phi[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1312,7 +1312,7 @@ __device__ __forceinline__ double rk_finish(const State &s, const TD *d24, size_
     }
     // the stage's state is streamed through once: non-temporal loads and
     // stores keep it out of the caches the gathers use
-    const double rr = r - (double)d24[o];
+    const double rr = r - (double)__builtin_nontemporal_load(&d24[o]);
     const double cu = __builtin_nontemporal_load(&s.cu[o]);
     double cn = __builtin_nontemporal_load(&s.cn[o]);
     if (stage == 1) {
@@ -1347,14 +1347,14 @@ __device__ __forceinline__ void source_and_update(const State &s, const Mesh &m,
         if (k >= 10) break;
         double r = rhs[j];
         switch (k) {
-        case 1: r = r - src[(size_t)SRC_MX * ne + i] - vis[(size_t)0 * ne + i]; break;
-        case 2: r = r - src[(size_t)SRC_MY * ne + i] - vis[(size_t)1 * ne + i]; break;
-        case 3: r = r - src[(size_t)SRC_MZ * ne + i] - vis[(size_t)2 * ne + i]; break;
-        case 4: r = r - src[(size_t)SRC_E * ne + i] - vis[(size_t)3 * ne + i]; break;
-        case 6: r = r + src[(size_t)SRC_MXP * ne + i] - vis[(size_t)4 * ne + i]; break;
-        case 7: r = r + src[(size_t)SRC_MY * ne + i] - vis[(size_t)5 * ne + i]; break;
-        case 8: r = r + src[(size_t)SRC_MZ * ne + i] - vis[(size_t)6 * ne + i]; break;
-        case 9: r = r + src[(size_t)SRC_E * ne + i] - vis[(size_t)7 * ne + i]; break;
+        case 1: r = r - __builtin_nontemporal_load(&src[(size_t)SRC_MX * ne + i]) - __builtin_nontemporal_load(&vis[(size_t)0 * ne + i]); break;
+        case 2: r = r - __builtin_nontemporal_load(&src[(size_t)SRC_MY * ne + i]) - __builtin_nontemporal_load(&vis[(size_t)1 * ne + i]); break;
+        case 3: r = r - __builtin_nontemporal_load(&src[(size_t)SRC_MZ * ne + i]) - __builtin_nontemporal_load(&vis[(size_t)2 * ne + i]); break;
+        case 4: r = r - __builtin_nontemporal_load(&src[(size_t)SRC_E * ne + i]) - __builtin_nontemporal_load(&vis[(size_t)3 * ne + i]); break;
+        case 6: r = r + __builtin_nontemporal_load(&src[(size_t)SRC_MXP * ne + i]) - __builtin_nontemporal_load(&vis[(size_t)4 * ne + i]); break;
+        case 7: r = r + __builtin_nontemporal_load(&src[(size_t)SRC_MY * ne + i]) - __builtin_nontemporal_load(&vis[(size_t)5 * ne + i]); break;
+        case 8: r = r + __builtin_nontemporal_load(&src[(size_t)SRC_MZ * ne + i]) - __builtin_nontemporal_load(&vis[(size_t)6 * ne + i]); break;
+        case 9: r = r + __builtin_nontemporal_load(&src[(size_t)SRC_E * ne + i]) - __builtin_nontemporal_load(&vis[(size_t)7 * ne + i]); break;
         default: break;
         }
         cv[j] = rk_finish(s, s.d24, (size_t)k * ne + i, r, stage, dtbyvol, store);
@@ -1416,10 +1416,11 @@ __device__ __forceinline__ void flux_update(const Params &P, const State &s, con
     const int neigh = m.nc[(size_t)f * ne + i];
     double F[10];
     {
-        const double scx = m.sc[(size_t)(3 * f) * ne + i], scy = m.sc[(size_t)(3 * f + 1) * ne + i],
-                     scz = m.sc[(size_t)(3 * f + 2) * ne + i];
-        const double xf = m.fc0[(size_t)(3 * f) * ne + i], yf = m.fc0[(size_t)(3 * f + 1) * ne + i],
-                     zf = m.fc0[(size_t)(3 * f + 2) * ne + i];
+        // the face's own-cell geometry is read once per stage (non-temporal)
+        const double scx = __builtin_nontemporal_load(&m.sc[(size_t)(3 * f) * ne + i]), scy = __builtin_nontemporal_load(&m.sc[(size_t)(3 * f + 1) * ne + i]),
+                     scz = __builtin_nontemporal_load(&m.sc[(size_t)(3 * f + 2) * ne + i]);
+        const double xf = __builtin_nontemporal_load(&m.fc0[(size_t)(3 * f) * ne + i]), yf = __builtin_nontemporal_load(&m.fc0[(size_t)(3 * f + 1) * ne + i]),
+                     zf = __builtin_nontemporal_load(&m.fc0[(size_t)(3 * f + 2) * ne + i]);
         const double area = sqrt(scx * scx + scy * scy + scz * scz);
         const double enx = scx / area, eny = scy / area, enz = scz / area;
         const int ntype = (neigh >= ne) ? m.gtype[neigh - ne] : 0;
